@@ -1,0 +1,200 @@
+/*
+ * pdp3_hip.h -- C ABI of the MI355X (gfx950) implementation of PDP3's r-z
+ * particle-in-cell step loop.
+ *
+ * One opaque context owns the geometry, the Yee grid (E, H, H at half time,
+ * J, rho of the bunches, PML conductivity), the species list and the model
+ * time, all resident in HBM, plus one HIP stream on which every call is
+ * enqueued.  Host arrays passed in are copied (borrowed for the call only);
+ * nothing but the handle crosses the boundary.  Every function returns 0 on
+ * success or a negative PDP3_ERR_* code; pdp3_last_error() gives the text.
+ * The library never exits the process (the reference exit()s or abort()s).
+ *
+ * Each entry point names the reference interface it replaces
+ * (cosmonaut-ok/pdp3, path:line).  Grid arrays are row-major FP64 with element
+ * (i, k) at i * n_grid_z + k, i radial, k axial, as the reference's
+ * `double **a; a[i][k]` (include/field.h:8-10).
+ *
+ * Not re-entrant per context: one host thread drives one context.
+ */
+#ifndef PDP3_HIP_H
+#define PDP3_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PDP3_OK 0
+#define PDP3_ERR_BAD_ARG (-1)
+#define PDP3_ERR_HIP (-2)
+#define PDP3_ERR_OOM (-3)
+#define PDP3_ERR_NONFINITE (-4)    /* NaN/inf particle position (src/particles.cpp:1046-1053) */
+#define PDP3_ERR_SUPERLUMINAL (-5) /* |v| > c in the pusher (src/lib.cpp:54-59) */
+#define PDP3_ERR_STATE (-6)
+
+/* ---- precision forms ---- */
+#define PDP3_PREC_FP64 0 /* plain form: FP64 throughout, reference arithmetic order */
+#define PDP3_PREC_FP32 1 /* FP32 push/deposit on int cell + FP32 offset, FP64 grid */
+
+/* ---- pushers (compile-time in the reference, configure.ac:142-154) ---- */
+#define PDP3_PUSHER_BORIS 0 /* PUSHER_BORIS_RELATIVISTIC (the default adaptive build compiles to it) */
+#define PDP3_PUSHER_VAY 1   /* PUSHER_VAY */
+
+/* ---- species kinds ---- */
+#define PDP3_SPECIES_PLASMA 0 /* Particles: specular walls (src/particles.cpp:1318-1353) */
+#define PDP3_SPECIES_BUNCH 1  /* Bunch: absorbed at r_max / z_max (src/bunch.cpp:126-153) */
+
+/* ---- grid components ---- */
+#define PDP3_E_R 0
+#define PDP3_E_PHI 1
+#define PDP3_E_Z 2
+#define PDP3_H_R 3
+#define PDP3_H_PHI 4
+#define PDP3_H_Z 5
+#define PDP3_HH_R 6 /* HField::field_r_half_time (include/hField.h:15-17) */
+#define PDP3_HH_PHI 7
+#define PDP3_HH_Z 8
+#define PDP3_J_R 9
+#define PDP3_J_PHI 10
+#define PDP3_J_Z 11
+#define PDP3_RHO_BUNCH 12 /* LoadInitParam::c_rho_bunch */
+#define PDP3_SIGMA 13     /* Geometry::sigma */
+#define PDP3_EPSILON 14   /* Geometry::epsilon */
+#define PDP3_NCOMP 15
+
+/* ---- kernels with live timing (pdp3_profile_get) ---- */
+#define PDP3_KERNEL_PARTICLES 0 /* fused push + deposit, plasma species */
+#define PDP3_KERNEL_H 1
+#define PDP3_KERNEL_E 2
+#define PDP3_KERNEL_SORT 3
+#define PDP3_KERNEL_BUNCH 4 /* fused push + deposit + charge weighting, bunches */
+#define PDP3_NKERNELS 5
+
+typedef struct pdp3_ctx pdp3_ctx;
+
+/* Geometry(double rs, double zs, int ngr, int ngz) + Time(ct, st, rt, et, dt)
+ * (include/geometry.h:20-24, src/geometry.cpp:8-31, src/pdp3Time.cpp:11-18).
+ * epsilon is set to 1 (Geometry::set_epsilon, src/geometry.cpp:33-39), all
+ * fields and sigma to 0. */
+int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size,
+                    double current_time, double start_time, double relaxation_time,
+                    double end_time, double delta_t, int pusher, int precision,
+                    pdp3_ctx **out);
+int pdp3_ctx_destroy(pdp3_ctx *ctx);
+const char *pdp3_last_error(const pdp3_ctx *ctx);
+/* geometry and time read-back: out[0..5] = dr, dz, r_size, z_size, n_grid_r, n_grid_z */
+int pdp3_geometry_get(const pdp3_ctx *ctx, double out[6]);
+/* Time members (include/pdp3Time.h:6-11) */
+int pdp3_time_get(const pdp3_ctx *ctx, double *current_time, double *delta_t);
+int pdp3_time_set(pdp3_ctx *ctx, double current_time);
+
+/* Geometry::set_pml(l1, l2, l3, sigma1, sigma2), src/geometry.cpp:56-107 */
+int pdp3_geometry_set_pml(pdp3_ctx *ctx, double l1, double l2, double l3,
+                          double sigma1, double sigma2);
+
+/* EField::boundary_conditions, src/eField.cpp:46-82 */
+int pdp3_efield_boundary_conditions(pdp3_ctx *ctx);
+/* EField::set_homogeneous_efield, src/eField.cpp:33-44 */
+int pdp3_efield_set_homogeneous(pdp3_ctx *ctx, double e_r, double e_phi, double e_z);
+/* HField::set_homogeneous_h, src/hField.cpp:66-91 */
+int pdp3_hfield_set_homogeneous(pdp3_ctx *ctx, double h_r, double h_phi, double h_z);
+/* BoundaryMaxwellConditions::specify_initial_field, src/boundaryMaxwellConditions.cpp:17-38 */
+int pdp3_boundary_specify_initial_field(pdp3_ctx *ctx, double e_phi_upper,
+                                        double e_phi_left, double e_phi_right);
+
+/* HField::calc_field(EField*, Time*), src/hField.cpp:93-157 */
+int pdp3_hfield_calc(pdp3_ctx *ctx);
+/* EField::calc_field(HField*, Time*, Current*), src/eField.cpp:85-170 */
+int pdp3_efield_calc(pdp3_ctx *ctx);
+/* Current::reset_j, src/current.cpp:125-135 */
+int pdp3_current_reset(pdp3_ctx *ctx);
+/* ChargeDensity::reset_rho on the bunch density, src/chargeDensity.cpp:43-49 */
+int pdp3_rho_bunch_reset(pdp3_ctx *ctx);
+
+/* EField::get_field / HField::get_field at n points (src/eField.cpp:204-316,
+ * src/hField.cpp:159-267): out is n x 3 (r, phi, z components) */
+int pdp3_efield_get_field(pdp3_ctx *ctx, const double *r, const double *z, int n, double *out);
+int pdp3_hfield_get_field(pdp3_ctx *ctx, const double *r, const double *z, int n, double *out);
+
+/* grid array transfer (the reference's public double** members,
+ * include/field.h:8-10, include/current.h:23-25, include/chargeDensity.h:17,
+ * include/geometry.h:14-15).  n_grid_r * n_grid_z doubles. */
+int pdp3_grid_download(pdp3_ctx *ctx, int component, double *out);
+int pdp3_grid_upload(pdp3_ctx *ctx, int component, const double *in);
+
+/* Particles(name, charge, mass, number, geom), src/particles.cpp:23-55:
+ * charge_e / mass_e are in electron charges / masses (parameters.xml).  The
+ * state starts dead; upload it with pdp3_species_upload.  Species are kept in
+ * ParticlesList order (include/particlesList.h:14): plasma species in the
+ * order they are added. */
+int pdp3_species_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, int *out_index);
+/* Bunch(...) ctor + LoadInitParam::init_beam bookkeeping, src/bunch.cpp:6-80,
+ * src/loadInitParam.cpp:168-248.  r and w as the constructor leaves them
+ * (z = v = 0, dead); inject_u[i] in [0,1) is the uniform draw that places
+ * particle i at injection (src/bunch.cpp:106-117).  The bunch joins the
+ * species list at the first step with t >= bunch_number*(duration+hole). */
+int pdp3_bunch_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number,
+                   const double *r, const double *w, const double *inject_u,
+                   double duration, double hole_duration, double velocity,
+                   int bunch_number, int *out_index);
+/* upload / download of a species' state.  w = real particles per
+ * macro-particle (charge_array = charge * w, src/particles.cpp:328-329);
+ * alive as Particles::is_alive.  Any pointer may be NULL (kept / skipped). */
+int pdp3_species_upload(pdp3_ctx *ctx, int index, const double *r, const double *z,
+                        const double *vr, const double *vphi, const double *vz,
+                        const double *w, const uint8_t *alive);
+int pdp3_species_download(pdp3_ctx *ctx, int index, double *r, double *z, double *vr,
+                          double *vphi, double *vz, double *w, uint8_t *alive);
+int pdp3_species_count(const pdp3_ctx *ctx, int index, int *number);
+/* synthetic uniform-in-cell plasma with an isotropic Maxwellian (counter-
+ * based Philox-4x32-10 draws keyed on (seed, species, cell, slot), so any
+ * device regenerates the same state).  Density is n0 for z-cells >= ramp_cells
+ * and ramps linearly from 0 at z = 0 before that (ramp_cells = 0: uniform).
+ * The species must have number == ppc * (n_grid_r-1) * (n_grid_z-1). */
+int pdp3_species_init_uniform(pdp3_ctx *ctx, int index, uint64_t seed, int ppc,
+                              double density, int ramp_cells, double temperature_ev);
+
+/* ParticlesList::step_v -> Particles::step_v for one species,
+ * src/particlesList.cpp:20-24, src/particles.cpp:1296-1316 */
+int pdp3_particles_step_v(pdp3_ctx *ctx, int index);
+/* Particles::full_current_distribution, src/particles.cpp:1367-1373 */
+int pdp3_particles_full_current_distribution(pdp3_ctx *ctx, int index);
+/* Particles::charge_weighting(ChargeDensity*) into the bunch density,
+ * src/particles.cpp:101-226 */
+int pdp3_particles_charge_weighting(pdp3_ctx *ctx, int index);
+/* Bunch::bunch_inject(Time*), src/bunch.cpp:87-124 */
+int pdp3_bunch_inject(pdp3_ctx *ctx, int index);
+
+/* nsteps iterations of LoadInitParam::run's loop body,
+ * src/loadInitParam.cpp:471-587: init_beam, H, reset J and rho, push and
+ * deposit of every species (fused, one pass over each species' particles),
+ * E, charge weighting of the bunches, t += dt.  Asynchronous except for the
+ * error check at the end. */
+int pdp3_step(pdp3_ctx *ctx, int nsteps);
+/* waits for the context's stream and reports device-side errors */
+int pdp3_sync(pdp3_ctx *ctx);
+
+/* stable counting sort of every species by cell (no reference counterpart:
+ * keeps gathers and deposits local).  sort_interval > 0 makes pdp3_step sort
+ * every sort_interval steps; 0 disables. */
+int pdp3_sort_particles(pdp3_ctx *ctx);
+int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
+
+/* live timing with HIP events on the context's stream (no reference
+ * counterpart): enable, then read the accumulated device time in ms and the
+ * launch count of one kernel class */
+int pdp3_profile_enable(pdp3_ctx *ctx, int enable);
+int pdp3_profile_get(pdp3_ctx *ctx, int kernel, double *total_ms, long long *launches);
+int pdp3_profile_reset(pdp3_ctx *ctx);
+
+/* library version string */
+const char *pdp3_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

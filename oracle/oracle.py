@@ -53,6 +53,7 @@ def lib():
         L.or_sim_set_pml.argtypes = [C.c_void_p] + [C.c_double] * 5
         L.or_sim_set_pusher.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.or_sim_set_reverse_deposit.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_time.restype = C.c_double
         L.or_sim_time.argtypes = [C.c_void_p]
         L.or_sim_dr.restype = C.c_double
@@ -126,6 +127,9 @@ class Sim:
 
     def set_pml(self, l1, l2, l3, s1, s2):
         lib().or_sim_set_pml(self.h, l1, l2, l3, s1, s2)
+
+    def set_reverse_deposit(self, reverse=True):
+        lib().or_sim_set_reverse_deposit(self.h, 1 if reverse else 0)
 
     @property
     def time(self):

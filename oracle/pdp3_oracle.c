@@ -61,6 +61,7 @@ struct or_sim {
   int order[MAX_SPECIES]; /* species list order (ParticlesList::part_list) */
   int norder;
   int next_bunch; /* init_beam's current_bunch_number */
+  int reverse_deposit; /* test hook: serial deposits in reverse particle order */
 };
 
 #define AT(s, arr, i, k) ((s)->a[arr][(size_t)(i) * (size_t)(s)->nz + (size_t)(k)])
@@ -142,6 +143,7 @@ void or_sim_set_time(or_sim *s, double current_time, double start_time,
 
 void or_sim_set_pusher(or_sim *s, int pusher) { s->pusher = pusher; }
 void or_sim_set_threads(or_sim *s, int threads) { s->threads = threads < 1 ? 1 : threads; }
+void or_sim_set_reverse_deposit(or_sim *s, int reverse) { s->reverse_deposit = reverse != 0; }
 double or_sim_time(const or_sim *s) { return s->current_time; }
 double or_sim_dr(const or_sim *s) { return s->dr; }
 double or_sim_dz(const or_sim *s) { return s->dz; }
@@ -1138,9 +1140,11 @@ int or_sim_full_current_distribution(or_sim *s, int idx) {
   const double half_dt = s->dt / 2.;
   int status = OR_OK;
   /* azimuthal_current_distribution (serial, :688-784) */
-  for (int i = 0; i < p->n; i++)
+  for (int j = 0; j < p->n; j++) {
+    int i = s->reverse_deposit ? p->n - 1 - j : j;
     if (p->alive[i])
       or_azimuthal_deposit(s, p->q[i], p->r[i], p->z[i], p->vphi[i], s->a[OR_J_PHI]);
+  }
   /* move_half_reflect (:1355-1365) */
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(s->threads) schedule(static) if (s->threads > 1)
@@ -1166,8 +1170,10 @@ int or_sim_full_current_distribution(or_sim *s, int idx) {
   if (status) return status;
   /* current_distribution (serial, :476-686) */
   dep_ctx d = {s->dr, s->dz, s->dt, (size_t)s->nz, s->a[OR_J_R], s->a[OR_J_Z]};
-  for (int i = 0; i < p->n; i++)
+  for (int j = 0; j < p->n; j++) {
+    int i = s->reverse_deposit ? p->n - 1 - j : j;
     if (p->alive[i]) deposit_one(&d, p->q[i], p->r[i], p->z[i], p->r_old[i], p->z_old[i]);
+  }
   /* back_velocity_to_rz (:1005-1023) */
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(s->threads) schedule(static) if (s->threads > 1)

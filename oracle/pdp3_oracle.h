@@ -58,6 +58,9 @@ double *or_sim_field(or_sim *s, int comp);
 /* reference-faithful OpenMP split: push / move loops parallel, deposits serial.
  * threads <= 1 keeps everything serial. */
 void or_sim_set_threads(or_sim *s, int threads);
+/* test hook: run the serial deposit loops in reverse particle order, to
+ * measure how much the loop's result depends on the order of the grid sums */
+void or_sim_set_reverse_deposit(or_sim *s, int reverse);
 
 /* plasma species (specular walls).  charge_e / mass_e in units of the
  * electron charge / mass as in parameters.xml (src/particles.cpp:30-33).

@@ -1,0 +1,263 @@
+"""ctypes binding of the C ABI in include/pdp3_hip.h (libpdp3_hip.so).
+
+The HIP library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpdp3_hip.so")
+
+OK = 0
+ERR_BAD_ARG, ERR_HIP, ERR_OOM, ERR_NONFINITE, ERR_SUPERLUMINAL, ERR_STATE = -1, -2, -3, -4, -5, -6
+PREC_FP64, PREC_FP32 = 0, 1
+PUSHER_BORIS, PUSHER_VAY = 0, 1
+SPECIES_PLASMA, SPECIES_BUNCH = 0, 1
+COMPONENTS = {
+    "E_r": 0, "E_phi": 1, "E_z": 2, "H_r": 3, "H_phi": 4, "H_z": 5,
+    "HH_r": 6, "HH_phi": 7, "HH_z": 8, "J_r": 9, "J_phi": 10, "J_z": 11,
+    "rho_bunch": 12, "sigma": 13, "epsilon": 14,
+}
+KERNEL_PARTICLES, KERNEL_H, KERNEL_E, KERNEL_SORT, KERNEL_BUNCH = range(5)
+
+_D = C.POINTER(C.c_double)
+_U8 = C.POINTER(C.c_uint8)
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_double
+
+# name -> (restype, argtypes); also the list of symbols include/pdp3_hip.h declares
+SIGNATURES = {
+    "pdp3_ctx_create": (_I, [_I, _I, _F, _F, _F, _F, _F, _F, _F, _I, _I, C.POINTER(_P)]),
+    "pdp3_ctx_destroy": (_I, [_P]),
+    "pdp3_last_error": (C.c_char_p, [_P]),
+    "pdp3_geometry_get": (_I, [_P, _D]),
+    "pdp3_time_get": (_I, [_P, _D, _D]),
+    "pdp3_time_set": (_I, [_P, _F]),
+    "pdp3_geometry_set_pml": (_I, [_P, _F, _F, _F, _F, _F]),
+    "pdp3_efield_boundary_conditions": (_I, [_P]),
+    "pdp3_efield_set_homogeneous": (_I, [_P, _F, _F, _F]),
+    "pdp3_hfield_set_homogeneous": (_I, [_P, _F, _F, _F]),
+    "pdp3_boundary_specify_initial_field": (_I, [_P, _F, _F, _F]),
+    "pdp3_hfield_calc": (_I, [_P]),
+    "pdp3_efield_calc": (_I, [_P]),
+    "pdp3_current_reset": (_I, [_P]),
+    "pdp3_rho_bunch_reset": (_I, [_P]),
+    "pdp3_efield_get_field": (_I, [_P, _D, _D, _I, _D]),
+    "pdp3_hfield_get_field": (_I, [_P, _D, _D, _I, _D]),
+    "pdp3_grid_download": (_I, [_P, _I, _D]),
+    "pdp3_grid_upload": (_I, [_P, _I, _D]),
+    "pdp3_species_add": (_I, [_P, _I, _F, _I, C.POINTER(_I)]),
+    "pdp3_bunch_add": (_I, [_P, _I, _F, _I, _D, _D, _D, _F, _F, _F, _I, C.POINTER(_I)]),
+    "pdp3_species_upload": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _U8]),
+    "pdp3_species_download": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _U8]),
+    "pdp3_species_count": (_I, [_P, _I, C.POINTER(_I)]),
+    "pdp3_species_init_uniform": (_I, [_P, _I, C.c_uint64, _I, _F, _I, _F]),
+    "pdp3_particles_step_v": (_I, [_P, _I]),
+    "pdp3_particles_full_current_distribution": (_I, [_P, _I]),
+    "pdp3_particles_charge_weighting": (_I, [_P, _I]),
+    "pdp3_bunch_inject": (_I, [_P, _I]),
+    "pdp3_step": (_I, [_P, _I]),
+    "pdp3_sync": (_I, [_P]),
+    "pdp3_sort_particles": (_I, [_P]),
+    "pdp3_set_sort_interval": (_I, [_P, _I]),
+    "pdp3_profile_enable": (_I, [_P, _I]),
+    "pdp3_profile_get": (_I, [_P, _I, _D, C.POINTER(C.c_longlong)]),
+    "pdp3_profile_reset": (_I, [_P]),
+    "pdp3_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+class Pdp3Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdp3 error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpdp3_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m pdp3_amd.build` (HIP library required)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Simulation:
+    """One pdp3 context: geometry, fields, species and time on the GPU."""
+
+    def __init__(self, n_grid_r, n_grid_z, r_size, z_size, dt, pusher=PUSHER_BORIS,
+                 precision=PREC_FP64, current_time=0.0, end_time=1e300):
+        self.L = lib()
+        h = _P()
+        st = self.L.pdp3_ctx_create(int(n_grid_r), int(n_grid_z), float(r_size), float(z_size), float(current_time),
+                                    0.0, 0.0, float(end_time), float(dt), int(pusher), int(precision), C.byref(h))
+        if st != OK:
+            raise Pdp3Error(st, "pdp3_ctx_create failed")
+        self.h = h
+        self.nr, self.nz = int(n_grid_r), int(n_grid_z)
+        g = np.empty(6)
+        self.L.pdp3_geometry_get(self.h, _dp(g))
+        self.dr, self.dz = float(g[0]), float(g[1])
+        self.dt = float(dt)
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pdp3_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st != OK:
+            raise Pdp3Error(st, self.L.pdp3_last_error(self.h).decode())
+        return st
+
+    # ---- geometry / fields ----
+    def set_pml(self, l1, l2, l3, s1, s2):
+        self._chk(self.L.pdp3_geometry_set_pml(self.h, l1, l2, l3, s1, s2))
+
+    def boundary_conditions(self):
+        self._chk(self.L.pdp3_efield_boundary_conditions(self.h))
+
+    @property
+    def time(self):
+        t = C.c_double()
+        self.L.pdp3_time_get(self.h, C.byref(t), None)
+        return t.value
+
+    def field(self, name):
+        comp = COMPONENTS[name] if isinstance(name, str) else int(name)
+        out = np.empty((self.nr, self.nz))
+        self._chk(self.L.pdp3_grid_download(self.h, comp, _dp(out)))
+        return out
+
+    def set_field(self, name, arr):
+        comp = COMPONENTS[name] if isinstance(name, str) else int(name)
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nr, self.nz)
+        self._chk(self.L.pdp3_grid_upload(self.h, comp, _dp(a)))
+
+    def h_calc(self):
+        self._chk(self.L.pdp3_hfield_calc(self.h))
+
+    def e_calc(self):
+        self._chk(self.L.pdp3_efield_calc(self.h))
+
+    def reset_j(self):
+        self._chk(self.L.pdp3_current_reset(self.h))
+
+    def get_e(self, r, z):
+        r, z = _f64(np.atleast_1d(r)), _f64(np.atleast_1d(z))
+        out = np.empty((len(r), 3))
+        self._chk(self.L.pdp3_efield_get_field(self.h, _dp(r), _dp(z), len(r), _dp(out)))
+        return out
+
+    def get_h(self, r, z):
+        r, z = _f64(np.atleast_1d(r)), _f64(np.atleast_1d(z))
+        out = np.empty((len(r), 3))
+        self._chk(self.L.pdp3_hfield_get_field(self.h, _dp(r), _dp(z), len(r), _dp(out)))
+        return out
+
+    # ---- species ----
+    def add_species(self, charge_e, mass_e, number):
+        idx = _I()
+        self._chk(self.L.pdp3_species_add(self.h, int(charge_e), float(mass_e), int(number), C.byref(idx)))
+        return idx.value
+
+    def add_plasma(self, charge_e, mass_e, r, z, vr, vphi, vz, w, alive=None):
+        idx = self.add_species(charge_e, mass_e, len(r))
+        self.upload(idx, r=r, z=z, vr=vr, vphi=vphi, vz=vz, w=w,
+                    alive=np.ones(len(r), np.uint8) if alive is None else alive)
+        return idx
+
+    def add_bunch(self, charge_e, mass_e, r, w, inject_u, duration, hole_duration, velocity, bunch_number=0):
+        r, w, u = _f64(r), _f64(w), _f64(inject_u)
+        idx = _I()
+        self._chk(self.L.pdp3_bunch_add(self.h, int(charge_e), float(mass_e), len(r), _dp(r), _dp(w), _dp(u),
+                                        float(duration), float(hole_duration), float(velocity), int(bunch_number),
+                                        C.byref(idx)))
+        return idx.value
+
+    def init_uniform(self, idx, seed, ppc, density, ramp_cells, temperature_ev):
+        self._chk(self.L.pdp3_species_init_uniform(self.h, idx, C.c_uint64(seed), int(ppc), float(density),
+                                                   int(ramp_cells), float(temperature_ev)))
+
+    def count(self, idx):
+        n = _I()
+        self._chk(self.L.pdp3_species_count(self.h, idx, C.byref(n)))
+        return n.value
+
+    def upload(self, idx, r=None, z=None, vr=None, vphi=None, vz=None, w=None, alive=None):
+        arrs = [_f64(a) for a in (r, z, vr, vphi, vz, w)]
+        al = None if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+        self._chk(self.L.pdp3_species_upload(self.h, idx, *(_dp(a) for a in arrs),
+                                             None if al is None else al.ctypes.data_as(_U8)))
+
+    def download(self, idx):
+        n = self.count(idx)
+        out = {k: np.empty(n) for k in ("r", "z", "vr", "vphi", "vz", "w")}
+        alive = np.empty(n, np.uint8)
+        self._chk(self.L.pdp3_species_download(self.h, idx, *(_dp(out[k]) for k in ("r", "z", "vr", "vphi", "vz", "w")),
+                                               alive.ctypes.data_as(_U8)))
+        out["alive"] = alive
+        return out
+
+    # ---- stepping ----
+    def step(self, n=1):
+        self._chk(self.L.pdp3_step(self.h, int(n)))
+
+    def step_v(self, idx):
+        self._chk(self.L.pdp3_particles_step_v(self.h, idx))
+
+    def full_current_distribution(self, idx):
+        self._chk(self.L.pdp3_particles_full_current_distribution(self.h, idx))
+
+    def charge_weighting(self, idx):
+        self._chk(self.L.pdp3_particles_charge_weighting(self.h, idx))
+
+    def sync(self):
+        self._chk(self.L.pdp3_sync(self.h))
+
+    def sort(self):
+        self._chk(self.L.pdp3_sort_particles(self.h))
+
+    def set_sort_interval(self, n):
+        self._chk(self.L.pdp3_set_sort_interval(self.h, int(n)))
+
+    # ---- timing ----
+    def profile(self, enable=True):
+        self._chk(self.L.pdp3_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_reset(self):
+        self._chk(self.L.pdp3_profile_reset(self.h))
+
+    def profile_get(self, kernel):
+        ms, n = C.c_double(), C.c_longlong()
+        self._chk(self.L.pdp3_profile_get(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

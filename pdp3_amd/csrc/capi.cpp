@@ -1,0 +1,785 @@
+// Host side of the C ABI (include/pdp3_hip.h): context and species
+// management, the step loop of LoadInitParam::run, bunch bookkeeping and live
+// kernel timing.  All device work is enqueued on the context's stream.
+#include "../../include/pdp3_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+struct pdp3_ctx {
+  pdp3::Ctx c;
+};
+
+using namespace pdp3;
+
+namespace {
+
+int fail(pdp3_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->c.last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? PDP3_ERR_OOM : PDP3_ERR_HIP,                    \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+  } while (0)
+
+#define CHECK_LAUNCH(ctx) HIP_TRY(ctx, hipGetLastError())
+
+bool valid_species(pdp3_ctx *ctx, int idx) {
+  return ctx && idx >= 0 && idx < (int)ctx->c.species.size();
+}
+
+// koef_e / koef_h of EField::calc_field (src/eField.cpp:105-110), per cell
+int upload_koef(pdp3_ctx *ctx) {
+  Ctx &c = ctx->c;
+  const size_t n = c.cells();
+  std::vector<double> ke(n), kh(n);
+  for (size_t p = 0; p < n; p++) {
+    double epsilonx2 = 2 * c.eps_host[p] * kEpsilon0;
+    double sigma_t = c.sigma_host[p] * c.g.dt;
+    ke[p] = (epsilonx2 - sigma_t) / (epsilonx2 + sigma_t);
+    kh[p] = 2 * c.g.dt / (epsilonx2 + sigma_t);
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(c.koef_e, ke.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.koef_h, kh.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kSigma], c.sigma_host.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                              c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kEps], c.eps_host.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                              c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+template <typename T>
+int dalloc(pdp3_ctx *ctx, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIP_TRY(ctx, hipMalloc((void **)p, n * sizeof(T)));
+  return PDP3_OK;
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+void free_species(Species &sp) {
+  for (auto *s : {&sp.f64, &sp.f64_alt}) {
+    dfree(s->r); dfree(s->z); dfree(s->vr); dfree(s->vphi); dfree(s->vz); dfree(s->w);
+    dfree(s->alive); dfree(s->r_old); dfree(s->z_old); dfree(s->sin_t);
+    *s = SpeciesF64{};
+  }
+  for (auto *s : {&sp.f32, &sp.f32_alt}) {
+    dfree(s->cell); dfree(s->fr); dfree(s->fz); dfree(s->vr); dfree(s->vphi); dfree(s->vz); dfree(s->w);
+    *s = SpeciesF32{};
+  }
+  dfree(sp.inject_u);
+  sp.inject_u = nullptr;
+}
+
+int alloc_species_state(pdp3_ctx *ctx, Species &sp) {
+  const size_t n = (size_t)sp.n;
+  int st;
+  if ((st = dalloc(ctx, &sp.f64.r, n)) || (st = dalloc(ctx, &sp.f64.z, n)) || (st = dalloc(ctx, &sp.f64.vr, n)) ||
+      (st = dalloc(ctx, &sp.f64.vphi, n)) || (st = dalloc(ctx, &sp.f64.vz, n)) ||
+      (st = dalloc(ctx, &sp.f64.w, n)) || (st = dalloc(ctx, &sp.f64.alive, n)))
+    return st;
+  HIP_TRY(ctx, hipMemsetAsync(sp.f64.alive, 0, n, ctx->c.stream));
+  for (double *p : {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w})
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, n * sizeof(double), ctx->c.stream));
+  return PDP3_OK;
+}
+
+int ensure_split_scratch(pdp3_ctx *ctx, Species &sp) {
+  if (sp.f64.r_old) return PDP3_OK;
+  int st;
+  if ((st = dalloc(ctx, &sp.f64.r_old, (size_t)sp.n)) || (st = dalloc(ctx, &sp.f64.z_old, (size_t)sp.n)) ||
+      (st = dalloc(ctx, &sp.f64.sin_t, (size_t)sp.n)))
+    return st;
+  return PDP3_OK;
+}
+
+// ---- live timing with HIP events on the context's stream ----
+struct TimedScope {
+  Ctx &c;
+  int kernel;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedScope(Ctx &cc, int k) : c(cc), kernel(k) {
+    if (!c.profiling) return;
+    KernelTimer &t = c.timers[kernel];
+    if (!t.pool.empty()) {
+      a = t.pool.back().first;
+      b = t.pool.back().second;
+      t.pool.pop_back();
+    } else {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+    }
+    (void)hipEventRecord(a, c.stream);
+  }
+  ~TimedScope() {
+    if (!c.profiling || !a) return;
+    (void)hipEventRecord(b, c.stream);
+    c.timers[kernel].pending.emplace_back(a, b);
+  }
+};
+
+void harvest_timers(Ctx &c) {
+  for (auto &t : c.timers) {
+    for (auto &ev : t.pending) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+        t.total_ms += ms;
+        t.launches++;
+      }
+      t.pool.push_back(ev);
+    }
+    t.pending.clear();
+  }
+}
+
+int check_device_errors(pdp3_ctx *ctx) {
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipMemcpyAsync(c.err_host, c.err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  harvest_timers(c);
+  int e = *c.err_host;
+  if (e) {
+    HIP_TRY(ctx, hipMemsetAsync(c.err, 0, sizeof(int), c.stream));
+    if (e & kErrNonfinite) return fail(ctx, PDP3_ERR_NONFINITE, "particle position is not a finite number");
+    if (e & kErrSuperluminal) return fail(ctx, PDP3_ERR_SUPERLUMINAL, "particle speed reached c (gamma complex)");
+    if (e & kErrDomain) return fail(ctx, PDP3_ERR_BAD_ARG, "particle outside the grid (deposit or gather dropped)");
+  }
+  return PDP3_OK;
+}
+
+bool is_f32(const Ctx &c) { return c.precision == PDP3_PREC_FP32; }
+
+int inject_bunch(pdp3_ctx *ctx, Species &sp) {
+  // Bunch::bunch_inject, src/bunch.cpp:87-124 (the index arithmetic runs here
+  // on the host in double exactly as in the reference)
+  Ctx &c = ctx->c;
+  double bunch_time_begin = sp.duration * sp.bunch_number + sp.hole_duration * sp.bunch_number;
+  double bunch_time_end = bunch_time_begin + sp.duration;
+  double dl = sp.velocity * c.g.dt;
+  int steps_amount = (int)std::ceil(sp.duration / c.g.dt);
+  if (steps_amount <= 0) return PDP3_OK;
+  unsigned particles_in_step = (unsigned)sp.n / (unsigned)steps_amount;
+  int start_number = (int)((c.current_time - bunch_time_begin) / c.g.dt * particles_in_step);
+  if (c.current_time >= bunch_time_begin && c.current_time < bunch_time_end) {
+    if (is_f32(c))
+      launch_bunch_inject_f32(c, sp, (unsigned)start_number, particles_in_step, dl);
+    else
+      launch_bunch_inject_f64(c, sp, (unsigned)start_number, particles_in_step, dl);
+    CHECK_LAUNCH(ctx);
+  }
+  return PDP3_OK;
+}
+
+// LoadInitParam::init_beam, src/loadInitParam.cpp:168-248
+int init_beam(pdp3_ctx *ctx) {
+  Ctx &c = ctx->c;
+  int cand = -1;
+  for (int j = 0; j < (int)c.species.size(); j++) {
+    Species &sp = c.species[j];
+    if (sp.kind == PDP3_SPECIES_BUNCH && !sp.created && sp.bunch_number == c.next_bunch) cand = j;
+  }
+  if (cand >= 0) {
+    Species &sp = c.species[cand];
+    double bunch_inject_time = c.next_bunch * (sp.duration + sp.hole_duration);
+    if (c.current_time >= bunch_inject_time) {
+      sp.created = true;
+      c.order.push_back(cand);
+      c.next_bunch++;
+    }
+  }
+  for (int j : c.order)
+    if (c.species[j].kind == PDP3_SPECIES_BUNCH) {
+      int st = inject_bunch(ctx, c.species[j]);
+      if (st) return st;
+    }
+  return PDP3_OK;
+}
+
+int zero_grid(pdp3_ctx *ctx, int comp) {
+  Ctx &c = ctx->c;
+  launch_fill(c.a[comp], c.cells(), 0.0, c.stream);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int particles_fused(pdp3_ctx *ctx, Species &sp) {
+  Ctx &c = ctx->c;
+  TimedScope ts(c, sp.kind == PDP3_SPECIES_BUNCH ? PDP3_KERNEL_BUNCH : PDP3_KERNEL_PARTICLES);
+  if (is_f32(c))
+    launch_fused_f32(c, sp);
+  else
+    launch_fused_f64(c, sp);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pdp3_version(void) { return "pdp3-mi355x 0.1 (gfx950)"; }
+
+int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, double current_time,
+                    double start_time, double relaxation_time, double end_time, double delta_t, int pusher,
+                    int precision, pdp3_ctx **out) {
+  if (!out) return PDP3_ERR_BAD_ARG;
+  *out = nullptr;
+  if (n_grid_r < 4 || n_grid_z < 4 || !(r_size > 0) || !(z_size > 0) || !(delta_t > 0)) return PDP3_ERR_BAD_ARG;
+  if (pusher != PDP3_PUSHER_BORIS && pusher != PDP3_PUSHER_VAY) return PDP3_ERR_BAD_ARG;
+  if (precision != PDP3_PREC_FP64 && precision != PDP3_PREC_FP32) return PDP3_ERR_BAD_ARG;
+  if ((long long)n_grid_r * n_grid_z >= (1LL << 31)) return PDP3_ERR_BAD_ARG;
+  pdp3_ctx *ctx = new (std::nothrow) pdp3_ctx();
+  if (!ctx) return PDP3_ERR_OOM;
+  Ctx &c = ctx->c;
+  c.g.nr = n_grid_r;
+  c.g.nz = n_grid_z;
+  c.g.r_size = r_size;
+  c.g.z_size = z_size;
+  c.g.dr = r_size / (n_grid_r - 1);  // Geometry::get_dr, src/geometry.cpp:45-48
+  c.g.dz = z_size / (n_grid_z - 1);
+  c.g.dt = delta_t;
+  c.current_time = current_time;
+  c.start_time = start_time;
+  c.relaxation_time = relaxation_time;
+  c.end_time = end_time;
+  c.pusher = pusher;
+  c.precision = precision;
+  int st = PDP3_OK;
+  auto bail = [&](int code) {
+    pdp3_ctx_destroy(ctx);
+    return code;
+  };
+  if (hipGetDevice(&c.device) != hipSuccess) return bail(PDP3_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(PDP3_ERR_HIP);
+  const size_t n = c.cells();
+  if ((st = dalloc(ctx, &c.grid, n * kNComp))) return bail(st);
+  for (int k = 0; k < kNComp; k++) c.a[k] = c.grid + (size_t)k * n;
+  if ((st = dalloc(ctx, &c.koef_e, n)) || (st = dalloc(ctx, &c.koef_h, n)) || (st = dalloc(ctx, &c.err, 1)))
+    return bail(st);
+  if (hipHostMalloc((void **)&c.err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    return bail(PDP3_ERR_HIP);
+  *c.err_host = 0;
+  if (hipMemsetAsync(c.grid, 0, n * kNComp * sizeof(double), c.stream) != hipSuccess) return bail(PDP3_ERR_HIP);
+  if (hipMemsetAsync(c.err, 0, sizeof(int), c.stream) != hipSuccess) return bail(PDP3_ERR_HIP);
+  c.sigma_host.assign(n, 0.0);
+  c.eps_host.assign(n, 1.0);  // Geometry::set_epsilon, src/geometry.cpp:33-39
+  if ((st = upload_koef(ctx))) return bail(st);
+  *out = ctx;
+  return PDP3_OK;
+}
+
+int pdp3_ctx_destroy(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_OK;
+  Ctx &c = ctx->c;
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  for (auto &sp : c.species) free_species(sp);
+  dfree(c.grid);
+  dfree(c.koef_e);
+  dfree(c.koef_h);
+  dfree(c.err);
+  dfree(c.sort_keys);
+  dfree(c.sort_perm);
+  dfree(c.sort_ws);
+  if (c.err_host) (void)hipHostFree(c.err_host);
+  for (auto &t : c.timers) {
+    for (auto &ev : t.pending) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    for (auto &ev : t.pool) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  }
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete ctx;
+  return PDP3_OK;
+}
+
+const char *pdp3_last_error(const pdp3_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+int pdp3_geometry_get(const pdp3_ctx *ctx, double out[6]) {
+  if (!ctx || !out) return PDP3_ERR_BAD_ARG;
+  const Ctx &c = ctx->c;
+  out[0] = c.g.dr;
+  out[1] = c.g.dz;
+  out[2] = c.g.r_size;
+  out[3] = c.g.z_size;
+  out[4] = c.g.nr;
+  out[5] = c.g.nz;
+  return PDP3_OK;
+}
+
+int pdp3_time_get(const pdp3_ctx *ctx, double *current_time, double *delta_t) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  if (current_time) *current_time = ctx->c.current_time;
+  if (delta_t) *delta_t = ctx->c.g.dt;
+  return PDP3_OK;
+}
+
+int pdp3_time_set(pdp3_ctx *ctx, double current_time) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  ctx->c.current_time = current_time;
+  return PDP3_OK;
+}
+
+// Geometry::set_pml, src/geometry.cpp:56-107 (host, once)
+int pdp3_geometry_set_pml(pdp3_ctx *ctx, double l1, double l2, double l3, double sigma1, double sigma2) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  const double dr = c.g.dr, dz = c.g.dz, r_size = c.g.r_size, z_size = c.g.z_size;
+  const int nr = c.g.nr, nz = c.g.nz;
+  auto S = [&](int i, int k) -> double & { return c.sigma_host[(size_t)i * nz + k]; };
+  double len_left = dz * (std::floor(nz * l1));
+  double len_right = dz * (std::floor(nz * l2));
+  double len_ext = dr * (std::floor(nr * l3));
+  if ((l1 == 0) && (l2 == 0) && (l3 != 0)) {
+    for (int i = 0; i < nr; i++)
+      for (int k = 0; k < nz; k++)
+        if (r_size - dr * i <= len_ext)
+          S(i, k) = sigma1 + (sigma2 - sigma1) / std::pow(len_ext, 2) * std::pow((dr * (i + 1) - r_size + len_ext), 2);
+  } else {
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nr; i++) {
+        if (dz * k < len_left)
+          S(i, k) = sigma1 + (sigma2 - sigma1) / std::pow(len_left, 2) * std::pow(len_left - dz * k, 2);
+        if (z_size - dz * k < len_right)
+          S(i, k) = sigma1 + (sigma2 - sigma1) / std::pow(len_right, 2) *
+                                 std::pow(dz * (k + 1) - z_size + len_right, 2);
+      }
+  }
+  if (l3 != 0) {
+    for (int i = 0; i < nr; i++)
+      for (int k = 0; k < nz; k++)
+        if (r_size - dr * i <= len_ext)
+          if ((r_size - dr * (i + 1) < len_ext * dz * k / len_left) &&
+              (r_size - dr * i <= len_ext / len_right * (z_size - dz * k)))
+            S(i, k) = sigma1 + (sigma2 - sigma1) / std::pow(len_ext, 2) * std::pow(dr * (i + 1) - r_size + len_ext, 2);
+  }
+  return upload_koef(ctx);
+}
+
+int pdp3_efield_boundary_conditions(pdp3_ctx *ctx) {
+  // EField::boundary_conditions, src/eField.cpp:46-82: the tangential wall
+  // values are set once; no update ever writes them afterwards
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  const int nr = c.g.nr, nz = c.g.nz;
+  std::vector<double> er(c.cells()), ephi(c.cells()), ez(c.cells());
+  HIP_TRY(ctx, hipMemcpyAsync(er.data(), c.a[kER], c.cells() * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ephi.data(), c.a[kEPhi], c.cells() * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ez.data(), c.a[kEZ], c.cells() * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  for (int i = 0; i < nr - 1; i++) er[(size_t)i * nz] = er[(size_t)i * nz + nz - 1] = 0;
+  for (int k = 0; k < nz; k++) ephi[k] = ephi[(size_t)(nr - 1) * nz + k] = 0;
+  for (int i = 0; i < nr; i++) ephi[(size_t)i * nz] = ephi[(size_t)i * nz + nz - 1] = 0;
+  for (int k = 0; k < nz - 1; k++) ez[k] = ez[(size_t)(nr - 1) * nz + k] = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kER], er.data(), c.cells() * 8, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kEPhi], ephi.data(), c.cells() * 8, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kEZ], ez.data(), c.cells() * 8, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+namespace {
+// sets a[i][k] = v for i < ni, k < nk (a rectangle anchored at (0,0))
+int set_block(pdp3_ctx *ctx, int comp, int ni, int nk, double v) {
+  Ctx &c = ctx->c;
+  std::vector<double> h(c.cells());
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), c.a[comp], c.cells() * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  for (int i = 0; i < ni; i++)
+    for (int k = 0; k < nk; k++) h[(size_t)i * c.g.nz + k] = v;
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[comp], h.data(), c.cells() * 8, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+}  // namespace
+
+int pdp3_efield_set_homogeneous(pdp3_ctx *ctx, double e_r, double e_phi, double e_z) {
+  // EField::set_homogeneous_efield, src/eField.cpp:33-44
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  const int nr = ctx->c.g.nr, nz = ctx->c.g.nz;
+  int st;
+  if ((st = set_block(ctx, kER, nr - 1, nz - 1, e_r)) || (st = set_block(ctx, kEPhi, nr - 1, nz - 1, e_phi)) ||
+      (st = set_block(ctx, kEZ, nr - 1, nz - 1, e_z)))
+    return st;
+  return PDP3_OK;
+}
+
+int pdp3_hfield_set_homogeneous(pdp3_ctx *ctx, double h_r, double h_phi, double h_z) {
+  // HField::set_homogeneous_h, src/hField.cpp:66-91
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  const int nr = ctx->c.g.nr, nz = ctx->c.g.nz;
+  int st;
+  if ((st = set_block(ctx, kHR, nr, nz - 1, h_r)) || (st = set_block(ctx, kHHR, nr, nz - 1, h_r)) ||
+      (st = set_block(ctx, kHPhi, nr - 1, nz - 1, h_phi)) || (st = set_block(ctx, kHHPhi, nr - 1, nz - 1, h_phi)) ||
+      (st = set_block(ctx, kHZ, nr - 1, nz, h_z)) || (st = set_block(ctx, kHHZ, nr - 1, nz, h_z)))
+    return st;
+  return PDP3_OK;
+}
+
+int pdp3_boundary_specify_initial_field(pdp3_ctx *ctx, double e_phi_upper, double e_phi_left,
+                                        double e_phi_right) {
+  // BoundaryMaxwellConditions::specify_initial_field,
+  // src/boundaryMaxwellConditions.cpp:17-38
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  const int nr = c.g.nr, nz = c.g.nz;
+  std::vector<double> h(c.cells());
+  HIP_TRY(ctx, hipMemcpyAsync(h.data(), c.a[kEPhi], c.cells() * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  for (int i = 0; i < nr; i++) {
+    h[(size_t)i * nz] = e_phi_left;
+    h[(size_t)i * nz + nz - 1] = e_phi_right;
+  }
+  for (int k = 0; k < nz; k++) h[(size_t)(nr - 1) * nz + k] = e_phi_upper;
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[kEPhi], h.data(), c.cells() * 8, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_hfield_calc(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  TimedScope ts(ctx->c, PDP3_KERNEL_H);
+  launch_h_update(ctx->c);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_efield_calc(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  TimedScope ts(ctx->c, PDP3_KERNEL_E);
+  launch_e_update(ctx->c);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_current_reset(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  int st;
+  if ((st = zero_grid(ctx, kJR)) || (st = zero_grid(ctx, kJPhi)) || (st = zero_grid(ctx, kJZ))) return st;
+  return PDP3_OK;
+}
+
+int pdp3_rho_bunch_reset(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  return zero_grid(ctx, kRhoBunch);
+}
+
+int pdp3_efield_get_field(pdp3_ctx *ctx, const double *r, const double *z, int n, double *out) {
+  if (!ctx || n < 0 || (n > 0 && (!r || !z || !out))) return PDP3_ERR_BAD_ARG;
+  if (n == 0) return PDP3_OK;
+  Ctx &c = ctx->c;
+  double *d = nullptr;
+  HIP_TRY(ctx, hipMalloc(&d, sizeof(double) * 5 * (size_t)n));
+  (void)hipMemcpyAsync(d, r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream);
+  (void)hipMemcpyAsync(d + n, z, sizeof(double) * n, hipMemcpyHostToDevice, c.stream);
+  launch_gather_f64(c, 0, d, d + n, n, d + 2 * n);
+  (void)hipMemcpyAsync(out, d + 2 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c.stream);
+  hipError_t e = hipStreamSynchronize(c.stream);
+  dfree(d);
+  if (e != hipSuccess) return fail(ctx, PDP3_ERR_HIP, hipGetErrorString(e));
+  return check_device_errors(ctx);
+}
+
+int pdp3_hfield_get_field(pdp3_ctx *ctx, const double *r, const double *z, int n, double *out) {
+  if (!ctx || n < 0 || (n > 0 && (!r || !z || !out))) return PDP3_ERR_BAD_ARG;
+  if (n == 0) return PDP3_OK;
+  Ctx &c = ctx->c;
+  double *d = nullptr;
+  HIP_TRY(ctx, hipMalloc(&d, sizeof(double) * 5 * (size_t)n));
+  (void)hipMemcpyAsync(d, r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream);
+  (void)hipMemcpyAsync(d + n, z, sizeof(double) * n, hipMemcpyHostToDevice, c.stream);
+  launch_gather_f64(c, 1, d, d + n, n, d + 2 * n);
+  (void)hipMemcpyAsync(out, d + 2 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c.stream);
+  hipError_t e = hipStreamSynchronize(c.stream);
+  dfree(d);
+  if (e != hipSuccess) return fail(ctx, PDP3_ERR_HIP, hipGetErrorString(e));
+  return check_device_errors(ctx);
+}
+
+int pdp3_grid_download(pdp3_ctx *ctx, int component, double *out) {
+  if (!ctx || component < 0 || component >= kNComp || !out) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipMemcpyAsync(out, c.a[component], c.cells() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_grid_upload(pdp3_ctx *ctx, int component, const double *in) {
+  if (!ctx || component < 0 || component >= kNComp || !in) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (component == kSigma || component == kEps) {
+    auto &dst = component == kSigma ? c.sigma_host : c.eps_host;
+    std::memcpy(dst.data(), in, c.cells() * sizeof(double));
+    return upload_koef(ctx);
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[component], in, c.cells() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_species_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, int *out_index) {
+  if (!ctx || number < 0 || !(mass_e > 0)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species sp;
+  sp.kind = PDP3_SPECIES_PLASMA;
+  sp.charge_e = charge_e;
+  sp.mass_e = mass_e;
+  sp.charge = charge_e * kElCharge;  // src/particles.cpp:30-33
+  sp.mass = mass_e * kElMass;
+  sp.n = number;
+  sp.created = true;
+  int st = alloc_species_state(ctx, sp);
+  if (!st && is_f32(c)) st = 0;  // the FP32 arrays are allocated at upload / init
+  if (st) {
+    free_species(sp);
+    return st;
+  }
+  c.species.push_back(sp);
+  int idx = (int)c.species.size() - 1;
+  c.order.push_back(idx);
+  if (out_index) *out_index = idx;
+  return PDP3_OK;
+}
+
+int pdp3_bunch_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, const double *r, const double *w,
+                   const double *inject_u, double duration, double hole_duration, double velocity,
+                   int bunch_number, int *out_index) {
+  if (!ctx || number < 0 || !(mass_e > 0) || (number > 0 && (!r || !w || !inject_u)) || !(duration > 0) ||
+      bunch_number < 0)
+    return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species sp;
+  sp.kind = PDP3_SPECIES_BUNCH;
+  sp.charge_e = charge_e;
+  sp.mass_e = mass_e;
+  sp.charge = charge_e * kElCharge;
+  sp.mass = mass_e * kElMass;
+  sp.n = number;
+  sp.duration = duration;
+  sp.hole_duration = hole_duration;
+  sp.velocity = velocity;
+  sp.bunch_number = bunch_number;
+  sp.created = false;
+  int st = alloc_species_state(ctx, sp);
+  if (!st) st = dalloc(ctx, &sp.inject_u, (size_t)number);
+  if (st) {
+    free_species(sp);
+    return st;
+  }
+  const size_t b = sizeof(double) * (size_t)number;
+  if (number > 0) {
+    HIP_TRY(ctx, hipMemcpyAsync(sp.f64.r, r, b, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sp.f64.w, w, b, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(sp.inject_u, inject_u, b, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  c.species.push_back(sp);
+  int idx = (int)c.species.size() - 1;
+  if (is_f32(c)) {
+    launch_f32_from_f64(c, c.species[idx]);
+    CHECK_LAUNCH(ctx);
+  }
+  if (out_index) *out_index = idx;
+  return PDP3_OK;
+}
+
+int pdp3_species_count(const pdp3_ctx *ctx, int index, int *number) {
+  if (!ctx || index < 0 || index >= (int)ctx->c.species.size() || !number) return PDP3_ERR_BAD_ARG;
+  *number = ctx->c.species[index].n;
+  return PDP3_OK;
+}
+
+int pdp3_species_upload(pdp3_ctx *ctx, int index, const double *r, const double *z, const double *vr,
+                        const double *vphi, const double *vz, const double *w, const uint8_t *alive) {
+  if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species &sp = c.species[index];
+  const size_t b = sizeof(double) * (size_t)sp.n;
+  if (sp.n == 0) return PDP3_OK;
+  if (is_f32(c)) {
+    // keep the FP64 mirror coherent before overwriting parts of it
+    launch_f64_from_f32(c, sp);
+    CHECK_LAUNCH(ctx);
+  }
+  const double *src[6] = {r, z, vr, vphi, vz, w};
+  double *dst[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
+  for (int j = 0; j < 6; j++)
+    if (src[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyHostToDevice, c.stream));
+  if (alive) HIP_TRY(ctx, hipMemcpyAsync(sp.f64.alive, alive, (size_t)sp.n, hipMemcpyHostToDevice, c.stream));
+  if (is_f32(c)) {
+    launch_f32_from_f64(c, sp);
+    CHECK_LAUNCH(ctx);
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_species_download(pdp3_ctx *ctx, int index, double *r, double *z, double *vr, double *vphi, double *vz,
+                          double *w, uint8_t *alive) {
+  if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species &sp = c.species[index];
+  const size_t b = sizeof(double) * (size_t)sp.n;
+  if (sp.n == 0) return PDP3_OK;
+  if (is_f32(c)) {
+    launch_f64_from_f32(c, sp);
+    CHECK_LAUNCH(ctx);
+  }
+  double *dst[6] = {r, z, vr, vphi, vz, w};
+  const double *src[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
+  for (int j = 0; j < 6; j++)
+    if (dst[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyDeviceToHost, c.stream));
+  if (alive) HIP_TRY(ctx, hipMemcpyAsync(alive, sp.f64.alive, (size_t)sp.n, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_species_init_uniform(pdp3_ctx *ctx, int index, uint64_t seed, int ppc, double density, int ramp_cells,
+                              double temperature_ev) {
+  if (!valid_species(ctx, index) || ppc <= 0 || !(density >= 0) || ramp_cells < 0 || !(temperature_ev >= 0))
+    return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species &sp = c.species[index];
+  long long want = (long long)ppc * (c.g.nr - 1) * (c.g.nz - 1);
+  if (sp.kind != PDP3_SPECIES_PLASMA || want != sp.n)
+    return fail(ctx, PDP3_ERR_BAD_ARG, "species size must be ppc * (n_grid_r-1) * (n_grid_z-1)");
+  launch_init_uniform(c, sp, index, seed, ppc, density, ramp_cells, temperature_ev);
+  CHECK_LAUNCH(ctx);
+  if (is_f32(c)) {
+    launch_f32_from_f64(c, sp);
+    CHECK_LAUNCH(ctx);
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_particles_step_v(pdp3_ctx *ctx, int index) {
+  if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (is_f32(c)) return fail(ctx, PDP3_ERR_STATE, "split entry points run in the FP64 form only");
+  int st = ensure_split_scratch(ctx, c.species[index]);
+  if (st) return st;
+  launch_step_v_f64(c, c.species[index]);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_particles_full_current_distribution(pdp3_ctx *ctx, int index) {
+  if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (is_f32(c)) return fail(ctx, PDP3_ERR_STATE, "split entry points run in the FP64 form only");
+  if (!c.species[index].f64.r_old) return fail(ctx, PDP3_ERR_STATE, "step_v must run first");
+  launch_full_current_f64(c, c.species[index]);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_particles_charge_weighting(pdp3_ctx *ctx, int index) {
+  if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (is_f32(c))
+    launch_charge_weighting_f32(c, c.species[index]);
+  else
+    launch_charge_weighting_f64(c, c.species[index]);
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_bunch_inject(pdp3_ctx *ctx, int index) {
+  if (!valid_species(ctx, index) || ctx->c.species[index].kind != PDP3_SPECIES_BUNCH) return PDP3_ERR_BAD_ARG;
+  return inject_bunch(ctx, ctx->c.species[index]);
+}
+
+int pdp3_step(pdp3_ctx *ctx, int nsteps) {
+  if (!ctx || nsteps < 0) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  for (int n = 0; n < nsteps; n++) {
+    int st;
+    if (c.sort_interval > 0 && c.steps_done > 0 && c.steps_done % c.sort_interval == 0) {
+      if ((st = pdp3_sort_particles(ctx))) return st;
+    }
+    if ((st = init_beam(ctx))) return st;
+    if ((st = pdp3_hfield_calc(ctx))) return st;
+    if ((st = pdp3_current_reset(ctx))) return st;
+    if ((st = zero_grid(ctx, kRhoBunch))) return st;
+    for (int j : c.order)
+      if ((st = particles_fused(ctx, c.species[j]))) return st;
+    if ((st = pdp3_efield_calc(ctx))) return st;
+    c.current_time = c.current_time + c.g.dt;  // src/loadInitParam.cpp:586
+    c.steps_done++;
+  }
+  return check_device_errors(ctx);
+}
+
+int pdp3_sync(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  return check_device_errors(ctx);
+}
+
+int pdp3_sort_particles(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  TimedScope ts(c, PDP3_KERNEL_SORT);
+  for (int j : c.order) {
+    int st = sort_species(c, c.species[j]);
+    if (st) return fail(ctx, st, "sort failed");
+  }
+  CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval) {
+  if (!ctx || sort_interval < 0) return PDP3_ERR_BAD_ARG;
+  ctx->c.sort_interval = sort_interval;
+  return PDP3_OK;
+}
+
+int pdp3_profile_enable(pdp3_ctx *ctx, int enable) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  ctx->c.profiling = enable != 0;
+  return PDP3_OK;
+}
+
+int pdp3_profile_get(pdp3_ctx *ctx, int kernel, double *total_ms, long long *launches) {
+  if (!ctx || kernel < 0 || kernel >= PDP3_NKERNELS) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  harvest_timers(c);
+  if (total_ms) *total_ms = c.timers[kernel].total_ms;
+  if (launches) *launches = c.timers[kernel].launches;
+  return PDP3_OK;
+}
+
+int pdp3_profile_reset(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  harvest_timers(c);
+  for (auto &t : c.timers) {
+    t.total_ms = 0;
+    t.launches = 0;
+  }
+  return PDP3_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// Internal state of a pdp3 context and the launchers shared between the HIP
+// translation units.  Not part of the public ABI (include/pdp3_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pdp3 {
+
+// constants, include/constant.h:10-27 (the reference's rounded SI values)
+constexpr double kPi = 3.1415926535897932;
+constexpr double kEpsilon0 = 8.85E-12;
+constexpr double kElMass = 9.1E-31;
+constexpr double kElCharge = 1.6E-19;
+constexpr double kLightSpeed = 3.0E8;
+constexpr double kMagnConst = 1.26E-6;
+constexpr double kMnzl = 1e-15;
+
+// device error bits, reported by pdp3_sync / pdp3_step
+constexpr int kErrNonfinite = 1;
+constexpr int kErrSuperluminal = 2;
+constexpr int kErrDomain = 4;
+
+enum GridComp {
+  kER = 0, kEPhi, kEZ, kHR, kHPhi, kHZ, kHHR, kHHPhi, kHHZ,
+  kJR, kJPhi, kJZ, kRhoBunch, kSigma, kEps, kNComp
+};
+
+// geometry and step constants, passed by value to every kernel
+struct GridDesc {
+  int nr, nz;           // node counts n_grid_r, n_grid_z
+  double dr, dz;        // src/geometry.cpp:45-53
+  double r_size, z_size;
+  double dt;
+};
+
+// FP64 particle state of one species (structure of arrays)
+struct SpeciesF64 {
+  double *r = nullptr, *z = nullptr, *vr = nullptr, *vphi = nullptr, *vz = nullptr, *w = nullptr;
+  uint8_t *alive = nullptr;
+  // split-entry-point scratch: position before the push and sin(theta)
+  double *r_old = nullptr, *z_old = nullptr, *sin_t = nullptr;
+};
+
+// FP32 cell-relative state of one species: cell = i * nz + k (or -1 dead),
+// offsets in units of the cell size in (0, 1]
+struct SpeciesF32 {
+  int32_t *cell = nullptr;
+  float *fr = nullptr, *fz = nullptr, *vr = nullptr, *vphi = nullptr, *vz = nullptr, *w = nullptr;
+};
+
+struct Species {
+  int kind = 0;  // PDP3_SPECIES_PLASMA / PDP3_SPECIES_BUNCH
+  int charge_e = 0;
+  double mass_e = 0;
+  double charge = 0, mass = 0;  // per real particle, src/particles.cpp:30-33
+  int n = 0;
+  SpeciesF64 f64;
+  SpeciesF32 f32;
+  // sort scratch (second buffer of the same layout) and keys
+  SpeciesF64 f64_alt;
+  SpeciesF32 f32_alt;
+  // bunch
+  double *inject_u = nullptr;
+  double duration = 0, hole_duration = 0, velocity = 0;
+  int bunch_number = 0;
+  bool created = true;
+};
+
+struct KernelTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  double total_ms = 0;
+  long long launches = 0;
+};
+
+struct Ctx {
+  GridDesc g{};
+  double current_time = 0, start_time = 0, relaxation_time = 0, end_time = 0;
+  int pusher = 0;
+  int precision = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double *grid = nullptr;  // kNComp contiguous nr*nz arrays
+  double *a[kNComp] = {};
+  double *koef_e = nullptr, *koef_h = nullptr;
+  std::vector<double> sigma_host, eps_host;
+  std::vector<Species> species;
+  std::vector<int> order;  // ParticlesList order of created species
+  int next_bunch = 0;
+  int *err = nullptr;      // device error bits
+  int *err_host = nullptr; // pinned mirror
+  std::string last_error;
+  bool profiling = false;
+  KernelTimer timers[5];
+  int sort_interval = 0;
+  long long steps_done = 0;
+  // sort workspace
+  int32_t *sort_keys = nullptr;
+  int32_t *sort_perm = nullptr;
+  size_t sort_ws_bytes = 0;
+  void *sort_ws = nullptr;
+  size_t sort_cap = 0;
+  size_t cells() const { return (size_t)g.nr * (size_t)g.nz; }
+};
+
+// ---- launchers (each enqueues on ctx.stream) ----
+void launch_h_update(Ctx &c);
+void launch_e_update(Ctx &c);
+void launch_fill(double *p, size_t n, double v, hipStream_t s);
+
+// FP64 plain form
+void launch_fused_f64(Ctx &c, Species &sp);
+void launch_step_v_f64(Ctx &c, Species &sp);
+void launch_full_current_f64(Ctx &c, Species &sp);
+void launch_charge_weighting_f64(Ctx &c, Species &sp);
+void launch_gather_f64(Ctx &c, int which, const double *r, const double *z, int n, double *out);
+void launch_bunch_inject_f64(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
+
+// FP32 form
+void launch_fused_f32(Ctx &c, Species &sp);
+void launch_charge_weighting_f32(Ctx &c, Species &sp);
+void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
+void launch_f32_from_f64(Ctx &c, Species &sp);  // pack absolute FP64 state into cell form
+void launch_f64_from_f32(Ctx &c, Species &sp);  // unpack for download
+
+// synthetic loader
+void launch_init_uniform(Ctx &c, Species &sp, int species_index, uint64_t seed, int ppc,
+                         double density, int ramp_cells, double temperature_ev);
+
+// sort
+int sort_species(Ctx &c, Species &sp);
+
+}  // namespace pdp3
