@@ -1,0 +1,174 @@
+"""Benchmark of the r-z PIC step (push + deposit + FDTD) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
+
+Workload (default cfg3): 512 x 8192 cells, dr = dz = 2.5e-4 m,
+dt = 0.5 dr / c, PML 16 cells, electrons at 100 per cell (4.19e8) with a
+density ramping linearly from 0 to 1e18 m^-3 over the first 1000 z-cells,
+immobile ion background, Gaussian bunch (sigma_r 0.5 mm, sigma_z 1 mm, peak
+5e17 m^-3, 0.999 c, 1e6 macro-particles), Boris pusher.  Synthetic data
+(device Philox loader, seeded).
+
+Both precision forms run the same workload from the same initial state:
+the FP32 cell-relative form (the reported value) and the FP64 plain form
+(reference arithmetic).  One JSON line goes to stdout with particle-steps/s
+of the FP32 form, the FP64 form's rate, E_z error of FP32 against FP64, the
+roofline of the dominant kernel (timed live with HIP events on the
+library's stream) and the CPU oracle timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s (push+deposit+FDTD), % HBM peak; FP32 vs FP64 at E_z error"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle-step of the fused particle kernel: FP32 form
+# reads cell 4 + offsets 8 + u 12 + w 4 and writes 24; FP64 plain form reads
+# r,z 16 + v 24 + w 8 + alive 1 and writes 40 + 1
+BYTES_PER_PARTICLE = {"fp32": 52.0, "fp64": 90.0}
+# per cell and launch: gather of E and H-half (6 x 8 B) and the J flush (3 x 8 B)
+BYTES_PER_CELL = 72.0
+
+
+def run_form(wl, capi, w, precision, steps, warmup, sort_interval):
+    sim = wl.build_simulation(w, precision=precision, sort_interval=sort_interval)
+    sim.sync()
+    sim.step(warmup)
+    sim.sync()
+    sim.profile(True)
+    sim.profile_reset()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms, launches = sim.profile_get(capi.KERNEL_PARTICLES)
+    ms_b, _ = sim.profile_get(capi.KERNEL_BUNCH)
+    ms_h, _ = sim.profile_get(capi.KERNEL_H)
+    ms_e, _ = sim.profile_get(capi.KERNEL_E)
+    ms_s, ls = sim.profile_get(capi.KERNEL_SORT)
+    sim.profile(False)
+    ez = sim.field("E_z")
+    out = dict(wall_s=t1 - t0, particle_ms=ms, particle_launches=launches, bunch_ms=ms_b, h_ms=ms_h, e_ms=ms_e,
+               sort_ms=ms_s, sort_launches=ls)
+    sim.close()
+    return out, ez
+
+
+def cpu_baseline(wl, capi, w, threads):
+    """The oracle (FP64, reference-faithful: push/move loops threaded, deposits
+    serial as in the reference) on a bounded sample of the workload: the same
+    cells in r, 128 cells in z, the same ppc and density, 3 steps."""
+    from oracle import oracle as orc
+
+    s = wl.config(w.name, seed=w.seed)
+    s.cells_z = 128
+    s.bunches = []
+    for p in s.plasma:
+        p.ramp_cells = 0
+    sim = wl.build_simulation(s, precision=capi.PREC_FP64)
+    states = [sim.download(j) for j in range(len(s.plasma))]
+    sim.close()
+    ora = orc.Sim(s.n_grid_r, s.n_grid_z, s.r_size, s.z_size, s.dt, pusher=s.pusher, threads=threads)
+    pml = s.pml()
+    if pml:
+        ora.set_pml(*pml)
+    for p, st in zip(s.plasma, states):
+        ora.add_plasma(p.charge, p.mass, st["r"], st["z"], st["vr"], st["vphi"], st["vz"], st["w"], st["alive"])
+    n = sum(len(st["r"]) for st in states)
+    nsteps = 3
+    t0 = time.perf_counter()
+    ora.step(nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=n * nsteps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{s.cells_r}x{s.cells_z}-cell slab of the same grid spacing, {n} particles "
+                       f"({s.plasma[0].ppc} ppc), {nsteps} full steps, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg3")
+    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-form comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(instances and z-slabs would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    w = wl.config(a.config)
+    n_plasma = sum(w.plasma_count(p) for p in w.plasma)
+    cells = w.cells_r * w.cells_z
+    print(f"# workload {w.name}: {w.cells_r}x{w.cells_z} cells, {n_plasma} plasma particles, "
+          f"{sum(len(b.r) for b in w.bunches)} bunch particles, steps {a.steps}, warmup {a.warmup}",
+          file=sys.stderr, flush=True)
+
+    r32, ez32 = run_form(wl, capi, w, capi.PREC_FP32, a.steps, a.warmup, a.sort_interval)
+    print(f"# fp32: {r32}", file=sys.stderr, flush=True)
+    value = n_plasma * a.steps / r32["wall_s"]
+    ms_per_step = 1e3 * r32["wall_s"] / a.steps
+
+    fp64 = None
+    err = None
+    if not a.no_fp64:
+        r64, ez64 = run_form(wl, capi, w, capi.PREC_FP64, a.steps, a.warmup, a.sort_interval)
+        print(f"# fp64: {r64}", file=sys.stderr, flush=True)
+        v64 = n_plasma * a.steps / r64["wall_s"]
+        den = np.sqrt(np.mean(ez64 ** 2))
+        err = dict(E_z_rel_rms=float(np.sqrt(np.mean((ez32 - ez64) ** 2)) / den),
+                   E_z_rel_max=float(np.max(np.abs(ez32 - ez64)) / np.max(np.abs(ez64))),
+                   after_steps=a.steps + a.warmup, reference="FP64 plain form on the GPU (oracle-parity checked)")
+        p64 = r64["particle_ms"] / max(r64["particle_launches"], 1)
+        fp64 = dict(value=v64, ms_per_step=1e3 * r64["wall_s"] / a.steps, particle_kernel_ms=p64,
+                    roofline_frac=(n_plasma * BYTES_PER_PARTICLE["fp64"] + cells * BYTES_PER_CELL) / (p64 * 1e-3)
+                    / 1e9 / HBM_PEAK_GBS)
+
+    per_launch_ms = r32["particle_ms"] / max(r32["particle_launches"], 1)
+    bytes_launch = n_plasma * BYTES_PER_PARTICLE["fp32"] + cells * BYTES_PER_CELL
+    achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel="k_fused32 (plasma species)", bytes_per_launch=bytes_launch,
+                    avg_launch_ms=per_launch_ms)
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, capi, w, threads)
+        print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{w.name}: r-z {w.cells_r}x{w.cells_z} cells, dr=dz={w.dr} m, "
+                               f"{w.plasma[0].ppc} ppc electrons ({n_plasma}) with a 0->1e18 m^-3 ramp over "
+                               f"1000 z-cells, Gaussian 0.999c bunch, PML {w.pml_cells}, Boris",
+                   "precision_form": "fp32 cell-relative (FP64 grid sums, axis cell, fields)",
+                   "sort_interval": a.sort_interval},
+        "fp64_plain": fp64,
+        "speedup_fp32_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "error_fp32_vs_fp64": err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -178,9 +178,11 @@ int pdp3_step(pdp3_ctx *ctx, int nsteps);
 /* waits for the context's stream and reports device-side errors */
 int pdp3_sync(pdp3_ctx *ctx);
 
-/* stable counting sort of every species by cell (no reference counterpart:
- * keeps gathers and deposits local).  sort_interval > 0 makes pdp3_step sort
- * every sort_interval steps; 0 disables. */
+/* stable counting sort of every plasma species by cell (no reference
+ * counterpart: keeps gathers and deposits local; bunches keep their index
+ * order, which their injection relies on).  Particle indices change: download
+ * after a sort returns the sorted order.  sort_interval > 0 makes pdp3_step
+ * sort every sort_interval steps; 0 disables. */
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
 

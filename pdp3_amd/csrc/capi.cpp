@@ -740,9 +740,11 @@ int pdp3_sort_particles(pdp3_ctx *ctx) {
   if (!ctx) return PDP3_ERR_BAD_ARG;
   Ctx &c = ctx->c;
   TimedScope ts(c, PDP3_KERNEL_SORT);
+  // bunches are injected by particle index (src/bunch.cpp:106-117) and stay unsorted
   for (int j : c.order) {
+    if (c.species[j].kind != PDP3_SPECIES_PLASMA) continue;
     int st = sort_species(c, c.species[j]);
-    if (st) return fail(ctx, st, "sort failed");
+    if (st) return fail(ctx, st == -3 ? PDP3_ERR_OOM : PDP3_ERR_HIP, "sort failed");
   }
   CHECK_LAUNCH(ctx);
   return PDP3_OK;

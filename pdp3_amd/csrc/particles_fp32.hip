@@ -1,10 +1,598 @@
-// FP32 cell-relative form (placeholder until the kernels land).
+// FP32 cell-relative form of the per-particle work of one step.
+//
+// State per particle: packed cell (bit 31 = dead, bits 16..30 = radial cell
+// i, bits 0..15 = axial cell k), offsets a, b in (0, 1] with r = (i + a) dr,
+// z = (k + b) dz (the reference's half-open cells (i dx, (i+1) dx],
+// include/geometry.h:44), momentum per mass u = gamma v (FP32), weight w.
+// Storing offsets instead of absolute coordinates keeps every bit of
+// resolution however large the grid is; storing u keeps gamma well
+// conditioned near c.
+//
+// Arithmetic: push and deposit in FP32; every formula of the reference is
+// rewritten in cell units without catastrophic cancellation (ring-volume
+// weights as products (1-a)(2i+a), the segment integrals of
+// simple_current_distribution through log1p-type series, the r-z rotation
+// through dy^2/(R+X)); the near-axis cell (1/r, the CYL_VOL annulus weights)
+// is evaluated in FP64; the grid accumulation is FP64 (hardware FP64
+// atomics); fields and the PML stay FP64.
 #include "context.hpp"
 
 namespace pdp3 {
-void launch_fused_f32(Ctx &, Species &) {}
-void launch_charge_weighting_f32(Ctx &, Species &) {}
-void launch_bunch_inject_f32(Ctx &, Species &, unsigned, unsigned, double) {}
-void launch_f32_from_f64(Ctx &, Species &) {}
-void launch_f64_from_f32(Ctx &, Species &) {}
+
+namespace {
+
+constexpr uint32_t kDead = 0x80000000u;
+
+__device__ __forceinline__ int cell_i(uint32_t c) { return (int)((c >> 16) & 0x7FFF); }
+__device__ __forceinline__ int cell_k(uint32_t c) { return (int)(c & 0xFFFF); }
+__device__ __forceinline__ uint32_t pack(int i, int k) { return ((uint32_t)i << 16) | (uint32_t)k; }
+
+struct F32Args {
+  int nr, nz;
+  float hr, hz;      // dt / (2 dr), dt / (2 dz): half-step displacement per unit speed, in cells
+  float inv_c2;      // 1 / c^2
+  float c1;          // q dt / (2 m) of the species (w cancels, src/particles.cpp:1060)
+  float bscale;      // Boris: mu0 c1; Vay: mu0 c1 * 2 c1^2 (the reference's Vay scaling, :1187-1210)
+  float escale;      // Boris: c1; Vay: 2 c1
+  float kz, kr, kp;  // charge/(4 pi dr^2 dt), charge/(2 pi dr dz dt), charge/(4 pi dz dr^2)
+  int pusher;
+  int is_bunch;
+  int want_rho;
+  int n;
+  const double *er, *ephi, *ez, *hhr, *hhphi, *hhz;
+  double *jr, *jphi, *jz, *rho;
+  int *err;
+  uint32_t *cell;
+  float *fa, *fb, *ur, *uphi, *uz;
+  const float *w;
+};
+
+__device__ __forceinline__ void add_node(const F32Args &p, double *a, int i, int k, double v) {
+  if ((unsigned)i < (unsigned)p.nr && (unsigned)k < (unsigned)p.nz)
+    unsafeAtomicAdd(a + (size_t)i * (size_t)p.nz + (size_t)k, v);
+  else
+    atomicOr(p.err, kErrDomain);
+}
+
+// radial weights of the four-node gathers (src/eField.cpp:204-316,
+// src/hField.cpp:159-267) in cell units.  "lo" family (E_r, H_z, H_phi):
+// staggered index from r - dr/2; otherwise (E_phi, E_z, H_r) the particle's
+// own cell with the CYL_VOL volume on the axis cell.  Weights A (node i_r),
+// B (node i_r+1), exactly the reference's ring volume / cell volume ratios
+// (which do not sum to one).
+struct RadW {
+  int ir;
+  float A, B;
+};
+__device__ __forceinline__ RadW rad_lo(int i, float a) {
+  int ir = i;
+  float al = a - 0.5f;
+  if (al <= 0.f) {
+    ir = i - 1;
+    al = a + 0.5f;
+  }
+  if (ir < 0) {  // clamped at the axis as the reference does
+    ir = 0;
+    al = (float)i + a - 0.5f;
+  }
+  float fi = (float)ir;
+  RadW w;
+  w.ir = ir;
+  w.A = (1.f - al) * (2.f * fi + 1.f + al) / (2.f * (fi + 1.f));
+  w.B = al * (2.f * fi + 2.f + al) / (2.f * (fi + 3.f));
+  return w;
+}
+__device__ __forceinline__ RadW rad_own(int i, float a) {
+  RadW w;
+  w.ir = i;
+  if (i == 0) {
+    double ad = a;  // axis cell in FP64: CYL_VOL annulus weight
+    w.A = (float)((1.0 - ad) * ad * 4.0);
+    w.B = (float)(ad * (1.0 + ad) / 4.0);
+  } else {
+    float fi = (float)i;
+    w.A = (1.f - a) * (2.f * fi + a) / (2.f * fi);
+    w.B = a * (2.f * fi + 1.f + a) / (2.f * (fi + 2.f));
+  }
+  return w;
+}
+struct AxW {
+  int kz;
+  float b;  // weight of node kz+1; node kz gets 1-b
+};
+__device__ __forceinline__ AxW ax_own(int k, float b) { return AxW{k, b}; }
+__device__ __forceinline__ AxW ax_half(int k, float b) {
+  int kz = k;
+  float be = b - 0.5f;
+  if (be <= 0.f) {
+    kz = k - 1;
+    be = b + 0.5f;
+  }
+  if (kz < 0) {
+    kz = 0;
+    be = (float)k + b - 0.5f;
+  }
+  return AxW{kz, be};
+}
+
+__device__ __forceinline__ float gather(const F32Args &p, const double *__restrict__ F, RadW rw, AxW aw) {
+  int ir = rw.ir, kz = aw.kz;
+  if (ir > p.nr - 2 || kz > p.nz - 2) {
+    atomicOr(p.err, kErrDomain);
+    ir = min(ir, p.nr - 2);
+    kz = min(kz, p.nz - 2);
+  }
+  const double *row0 = F + (size_t)ir * (size_t)p.nz + kz;
+  const double *row1 = row0 + p.nz;
+  float f00 = (float)row0[0], f01 = (float)row0[1], f10 = (float)row1[0], f11 = (float)row1[1];
+  float b = aw.b, ob = 1.f - aw.b;
+  return (f00 * ob + f01 * b) * rw.A + (f10 * ob + f11 * b) * rw.B;
+}
+
+__device__ __forceinline__ float dot3f(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// relativistic Boris on u = gamma v, src/particles.cpp:1025-1150
+__device__ __forceinline__ void boris_u(const F32Args &p, const float e[3], const float h[3], float u[3]) {
+  float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
+  float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
+  for (int j = 0; j < 3; j++) u[j] += ee[j];
+  float g = rsqrtf(1.f + dot3f(u, u) * p.inv_c2);
+  for (int j = 0; j < 3; j++) b[j] *= g;
+  float c2 = 2.f / (1.f + dot3f(b, b));
+  float t0 = u[0], t1 = u[1], t2 = u[2];
+  u[0] = t0 + c2 * ((t1 - t0 * b[2] + t2 * b[0]) * b[2] - (t2 + t0 * b[1] - t1 * b[0]) * b[1]);
+  u[1] = t1 + c2 * (-(t0 + t1 * b[2] - t2 * b[1]) * b[2] + (t2 + t0 * b[1] - t1 * b[0]) * b[0]);
+  u[2] = t2 + c2 * ((t0 + t1 * b[2] - t2 * b[1]) * b[1] - (t1 - t0 * b[2] + t2 * b[0]) * b[0]);
+  for (int j = 0; j < 3; j++) u[j] += ee[j];
+}
+
+// Vay on u, src/particles.cpp:1152-1254 (with the reference's b scaling)
+__device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const float h[3], float u[3]) {
+  float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
+  float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
+  float t[3] = {u[0], u[1], u[2]};
+  float g = rsqrtf(1.f + dot3f(t, t) * p.inv_c2);
+  float up[3];
+  up[0] = t[0] + ee[0] + g * (t[1] * b[2] - t[2] * b[1]);
+  up[1] = t[1] + ee[1] + g * (t[2] * b[0] - t[0] * b[2]);
+  up[2] = t[2] + ee[2] + g * (t[0] * b[1] - t[1] * b[0]);
+  float alpha = 1.f + dot3f(up, up) * p.inv_c2;  // gamma'^2
+  float B2 = dot3f(b, b);
+  float s = alpha - B2;
+  float ub = dot3f(up, b);
+  float us2 = ub * ub;
+  float ig = rsqrtf(0.5f * (s + sqrtf(s * s + 4.f * (B2 + us2))));  // 1/gamma^{n+1}
+  for (int j = 0; j < 3; j++) b[j] *= ig;
+  s = 1.f / (1.f + dot3f(b, b));
+  alpha = dot3f(up, b);
+  u[0] = s * (up[0] + alpha * b[0] + b[2] * up[1] - b[1] * up[2]);
+  u[1] = s * (up[1] + alpha * b[1] + b[0] * up[2] - b[2] * up[0]);
+  u[2] = s * (up[2] + alpha * b[2] + b[1] * up[0] - b[0] * up[1]);
+}
+
+// keep an offset in (0, 1] by moving at most one cell
+__device__ __forceinline__ void renorm(int &c, float &o) {
+  if (o > 1.f) {
+    o -= 1.f;
+    c += 1;
+  } else if (o <= 0.f) {
+    o += 1.f;
+    c -= 1;
+  }
+}
+
+// r of the Cartesian point (X, dy), X = i + xa, as an offset update that
+// never forms X^2 + dy^2 - X^2: sqrt(X^2+dy^2) - X = dy^2 / (R + X)
+__device__ __forceinline__ float rotate_back(int i, float &xa, float dy) {
+  float X = (float)i + xa;
+  float R = sqrtf(X * X + dy * dy);
+  xa += dy * dy / (R + X);
+  return dy / R;  // sin(theta)
+}
+
+// J_phi of one particle at the half-step position,
+// azimuthal_current_distribution (src/particles.cpp:688-784) in cell units
+__device__ void deposit_jphi(const F32Args &p, float w, int i, float a, int k, float b, float vphi) {
+  double c0, c1;  // node (i,k) and (i+1,k) weights before the axial split
+  if (i == 0) {
+    double ad = a;  // axis cell in FP64; rho = a here
+    c0 = 8.0 * (1.0 - ad);
+    c1 = 1.0 + ad;
+  } else {
+    float fi = (float)i, rho = fi + a;
+    c0 = (1.f - a) * (2.f * fi + a) / (rho * fi);
+    c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
+  }
+  float s = p.kp * w * vphi;
+  add_node(p, p.jphi, i, k, (double)(s * (1.f - b)) * c0);
+  add_node(p, p.jphi, i + 1, k, (double)(s * (1.f - b)) * c1);
+  add_node(p, p.jphi, i, k + 1, (double)(s * b) * c0);
+  add_node(p, p.jphi, i + 1, k + 1, (double)(s * b) * c1);
+}
+
+// Particles::charge_weighting of one particle (src/particles.cpp:101-226)
+__device__ void deposit_rho(const F32Args &p, float w, int i, float a, int k, float b) {
+  double c0, c1;
+  if (i == 0) {
+    double rho = a;
+    if (rho <= 0.5) {  // r <= dr/2 branch (:155-186)
+      c0 = 16.0 * (0.5 - rho + rho * rho) / (2.0 * rho * rho + 0.5);
+      c1 = 2.0 * rho * (rho + 1.0) / (2.0 * rho * rho + 0.5);
+    } else {  // dr/2 < r <= dr (:188-218)
+      c0 = 8.0 * (1.0 - rho);
+      c1 = 1.0 + rho;
+    }
+  } else {
+    float fi = (float)i, rho = fi + a;
+    c0 = (1.f - a) * (2.f * fi + a) / (rho * fi);
+    c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
+  }
+  float s = p.kp * w;
+  add_node(p, p.rho, i, k, (double)(s * (1.f - b)) * c0);
+  add_node(p, p.rho, i + 1, k, (double)(s * (1.f - b)) * c1);
+  add_node(p, p.rho, i, k + 1, (double)(s * b) * c0);
+  add_node(p, p.rho, i + 1, k + 1, (double)(s * b) * c1);
+}
+
+// (log(1+x) - x) / x without cancellation
+__device__ __forceinline__ float m_of(float x) {
+  if (fabsf(x) < 0.1f)
+    return x * (-0.5f + x * (1.f / 3.f + x * (-0.25f + x * (0.2f + x * (-1.f / 6.f + x * (1.f / 7.f - x * 0.125f))))));
+  return (log1pf(x) - x) / x;
+}
+
+// one straight piece of the path inside cell (i, k): simple_current_distribution
+// (src/particles.cpp:333-474) rewritten in cell units.  (a, b): start offsets
+// in the cell, (dr_, dz_): displacement in cells.  The closed forms
+//   J_z[i]   ~ dz/i     [ (1-a)(2i+a)/rho - dr/2 + i(i+1)/rho M(x) ]
+//   J_z[i+1] ~ dz/(i+1) [ a(2i+1+a)/rho  + dr/2 - i(i+1)/rho M(x) ]
+//   J_r[k]   ~ [dr((1-b)/2 - dz/4) + i(i+1)/(2rho)(dr(1-b)/rho_e + dz N(x))] / (i+1/2)
+//   J_r[k+1] ~ [dr(b/2 + dz/4)     + i(i+1)/(2rho)(dr b/rho_e     - dz N(x))] / (i+1/2)
+// with rho = i + a, x = dr/rho, M = (ln(1+x)-x)/x, N = 1/(1+x) - ln(1+x)/x are
+// algebraically identical to the reference's expressions (which evaluate them
+// as differences of O(i) terms); they also reduce to strict_motion_weighting
+// (:786-988) for purely axial or radial motion.
+__device__ void segment(const F32Args &p, float w, int i, int k, float a, float b, float drho, float dzeta) {
+  float jz0, jz1, jr0, jr1;
+  if (i == 0) {
+    double rs = a;  // axis cell in FP64
+    jz0 = (float)(8.0 * (1.0 - 0.5 * drho - rs));
+    jz1 = (float)(0.5 * drho + 1.0 + rs);
+    jr0 = drho * (0.5f * (1.f - b) - 0.25f * dzeta) * 2.f;
+    jr1 = drho * (0.5f * b + 0.25f * dzeta) * 2.f;
+  } else {
+    float fi = (float)i;
+    float rs = fi + a, re = rs + drho;
+    float x = drho / rs;
+    float M = m_of(x);
+    float N = -x / (1.f + x) - M;
+    float ii = fi * (fi + 1.f);
+    jz0 = ((1.f - a) * (2.f * fi + a) / rs - 0.5f * drho + ii / rs * M) / fi;
+    jz1 = (a * (2.f * fi + 1.f + a) / rs + 0.5f * drho - ii / rs * M) / (fi + 1.f);
+    float cc = ii / (2.f * rs);
+    float inv = 1.f / (fi + 0.5f);
+    jr0 = (drho * (0.5f * (1.f - b) - 0.25f * dzeta) + cc * (drho * (1.f - b) / re + dzeta * N)) * inv;
+    jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b / re - dzeta * N)) * inv;
+  }
+  float sz = p.kz * w * dzeta, sr = p.kr * w;
+  add_node(p, p.jz, i, k, (double)(sz * jz0));
+  add_node(p, p.jz, i + 1, k, (double)(sz * jz1));
+  add_node(p, p.jr, i, k, (double)(sr * jr0));
+  add_node(p, p.jr, i, k + 1, (double)(sr * jr1));
+}
+
+// current_distribution (src/particles.cpp:476-686): the straight path from
+// the old to the new position, split at the cell faces it crosses (at most
+// one radial and one axial face: |displacement| <= c dt <= dr/2), each piece
+// deposited in its own cell
+__device__ void deposit_j(const F32Args &p, float w, int io, float ao, int ko, float bo, int in_, float an,
+                          int kn, float bn) {
+  int di = in_ - io, dk = kn - ko;
+  if (di < -1 || di > 1 || dk < -1 || dk > 1) {
+    atomicOr(p.err, kErrDomain);
+    return;
+  }
+  float drho = (float)di + (an - ao);
+  float dzeta = (float)dk + (bn - bo);
+  // parameters along the path where it crosses the faces (in old-cell units
+  // the radial face is at offset 1 (di=+1) or 0 (di=-1))
+  float tr = 2.f, tz = 2.f;
+  if (di != 0) tr = ((di > 0 ? 1.f : 0.f) - ao) / drho;
+  if (dk != 0) tz = ((dk > 0 ? 1.f : 0.f) - bo) / dzeta;
+  int ci = io, ck = ko;
+  float ca = ao, cb = bo;  // current start point, offsets in cell (ci, ck)
+  float tprev = 0.f;
+  for (int pass = 0; pass < 2; pass++) {
+    bool rfirst = tr <= tz;
+    float t = rfirst ? tr : tz;
+    if (t > 1.f) break;
+    float sd_r = (t - tprev) * drho, sd_z = (t - tprev) * dzeta;
+    if (rfirst) {
+      float fr = di > 0 ? 1.f : 0.f;
+      segment(p, w, ci, ck, ca, cb, fr - ca, sd_z);
+      ci += di;
+      ca = di > 0 ? 0.f : 1.f;
+      cb += sd_z;
+      tr = 2.f;
+    } else {
+      float fz = dk > 0 ? 1.f : 0.f;
+      segment(p, w, ci, ck, ca, cb, sd_r, fz - cb);
+      ck += dk;
+      cb = dk > 0 ? 0.f : 1.f;
+      ca += sd_r;
+      tz = 2.f;
+    }
+    tprev = t;
+  }
+  // last piece to the new position
+  segment(p, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb);
+}
+
+// wall handling of move_half_reflect on the Cartesian half-step point
+// (src/particles.cpp:1318-1353, src/bunch.cpp:126-153), in cell units: the
+// walls sit at offset 1/2 of the first and last cells
+__device__ __forceinline__ void walls(const F32Args &p, int &i, float &xa, int &k, float &b, float u[3],
+                                      bool &alive) {
+  const int il = p.nr - 2, kl = p.nz - 2;
+  float off = (float)(i - il) + xa;
+  if (i >= il - 1 && off > 0.5f) {
+    if (p.is_bunch) {
+      alive = false;
+    } else {
+      i = il;
+      xa = 1.f - off;
+      u[0] = -u[0];
+    }
+  }
+  float offz = (float)(k - kl) + b;
+  if (k >= kl - 1 && offz > 0.5f) {
+    if (p.is_bunch) {
+      alive = false;
+    } else {
+      k = kl;
+      b = 1.f - offz;
+      u[2] = -u[2];
+    }
+  }
+  if (i <= 1) {
+    float o0 = (float)i + xa;
+    if (o0 < 0.5f) {
+      i = 0;
+      xa = 1.f - o0;
+      u[0] = -u[0];
+    }
+  }
+  if (k <= 1) {
+    float o0 = (float)k + b;
+    if (o0 < 0.5f) {
+      k = 0;
+      b = 1.f - o0;
+      u[2] = -u[2];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fused32(F32Args p) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.n) return;
+  uint32_t c = p.cell[n];
+  if (c & kDead) return;
+  int i = cell_i(c), k = cell_k(c);
+  float a = p.fa[n], b = p.fb[n];
+  float u[3] = {p.ur[n], p.uphi[n], p.uz[n]};
+  const float w = p.w[n];
+  // gathers (E at the full step, H at the half-time copy)
+  RadW rl = rad_lo(i, a), ro = rad_own(i, a);
+  AxW zo = ax_own(k, b), zh = ax_half(k, b);
+  float e[3], h[3];
+  e[0] = gather(p, p.er, rl, zo);
+  e[1] = gather(p, p.ephi, ro, zo);
+  e[2] = gather(p, p.ez, ro, zh);
+  h[0] = gather(p, p.hhr, ro, zh);
+  h[1] = gather(p, p.hhphi, rl, zh);
+  h[2] = gather(p, p.hhz, rl, zo);
+  if (p.pusher == 1)
+    vay_u(p, e, h, u);
+  else
+    boris_u(p, e, h, u);
+  float g = rsqrtf(1.f + dot3f(u, u) * p.inv_c2);
+  float dx = u[0] * g * p.hr, dy = u[1] * g * p.hr, dz = u[2] * g * p.hz;
+  // first half step and rotation back to r-z (step_v, :1296-1316)
+  int i1 = i, k1 = k;
+  float xa = a + dx, b1 = b + dz;
+  rotate_back(i1, xa, dy);
+  renorm(i1, xa);
+  renorm(k1, b1);
+  deposit_jphi(p, w, i1, xa, k1, b1, u[1] * g);
+  // second half step with the walls (move_half_reflect, :1355-1365)
+  float xa2 = xa + dx, b2 = b1 + dz;
+  bool alive = true;
+  walls(p, i1, xa2, k1, b2, u, alive);
+  float st = rotate_back(i1, xa2, dy);
+  renorm(i1, xa2);
+  renorm(k1, b2);
+  if (alive) {
+    deposit_j(p, w, i, a, k, b, i1, xa2, k1, b2);
+    // back_velocity_to_rz with the second half step's angle (:1005-1023)
+    float ct = sqrtf(1.f - st * st);
+    float u0 = ct * u[0] - st * u[1];
+    float u1 = st * u[0] + ct * u[1];
+    u[0] = u0;
+    u[1] = u1;
+    if (p.want_rho) deposit_rho(p, w, i1, xa2, k1, b2);
+  }
+  p.cell[n] = pack(i1, k1) | (alive ? 0u : kDead);
+  p.fa[n] = xa2;
+  p.fb[n] = b2;
+  p.ur[n] = u[0];
+  p.uphi[n] = u[1];
+  p.uz[n] = u[2];
+}
+
+__global__ void k_charge32(F32Args p) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.n) return;
+  uint32_t c = p.cell[n];
+  if (c & kDead) return;
+  deposit_rho(p, p.w[n], cell_i(c), p.fa[n], cell_k(c), p.fb[n]);
+}
+
+// absolute FP64 state -> cell form (cell index by CELL_NUMBER clamped at 0)
+__global__ void k_pack(GridDesc g, const double *r, const double *z, const double *vr, const double *vphi,
+                       const double *vz, const double *w, const uint8_t *alive, int n, uint32_t *cell, float *fa,
+                       float *fb, float *ur, float *uphi, float *uz, float *fw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double xr = r[j] / g.dr, xz = z[j] / g.dz;
+  int i = (int)ceil(xr) - 1, k = (int)ceil(xz) - 1;
+  if (i < 0) i = 0;
+  if (k < 0) k = 0;
+  if (i > g.nr - 2) i = g.nr - 2;
+  if (k > g.nz - 2) k = g.nz - 2;
+  float oa = (float)(xr - i), ob = (float)(xz - k);
+  double v2 = vr[j] * vr[j] + vphi[j] * vphi[j] + vz[j] * vz[j];
+  double gam = 1.0 / sqrt(1.0 - v2 / (kLightSpeed * kLightSpeed));
+  cell[j] = pack(i, k) | (alive[j] ? 0u : kDead);
+  fa[j] = oa;
+  fb[j] = ob;
+  ur[j] = (float)(gam * vr[j]);
+  uphi[j] = (float)(gam * vphi[j]);
+  uz[j] = (float)(gam * vz[j]);
+  fw[j] = (float)w[j];
+}
+
+__global__ void k_unpack(GridDesc g, double *r, double *z, double *vr, double *vphi, double *vz, double *w,
+                         uint8_t *alive, int n, const uint32_t *cell, const float *fa, const float *fb,
+                         const float *ur, const float *uphi, const float *uz, const float *fw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t c = cell[j];
+  r[j] = ((double)cell_i(c) + (double)fa[j]) * g.dr;
+  z[j] = ((double)cell_k(c) + (double)fb[j]) * g.dz;
+  double u0 = ur[j], u1 = uphi[j], u2 = uz[j];
+  double ig = 1.0 / sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2) / (kLightSpeed * kLightSpeed));
+  vr[j] = u0 * ig;
+  vphi[j] = u1 * ig;
+  vz[j] = u2 * ig;
+  w[j] = fw[j];
+  alive[j] = (c & kDead) ? 0 : 1;
+}
+
+// Bunch::bunch_inject (src/bunch.cpp:104-121) on the cell form: z from the
+// draw, v_z set to the bunch speed, the radial state kept
+__global__ void k_inject32(GridDesc g, uint32_t *cell, float *fb, float *ur, float *uphi, float *uz,
+                           const double *u, int n, unsigned start, unsigned count, double dl, double velocity) {
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  unsigned j = t + start;
+  if (j >= (unsigned)n) return;
+  double zabs = dl * u[j] + g.dz / 2.0;
+  double xz = zabs / g.dz;
+  int k = (int)ceil(xz) - 1;
+  if (k < 0) k = 0;
+  uint32_t c = cell[j];
+  double u0 = ur[j], u1 = uphi[j], u2 = uz[j];
+  double ig = 1.0 / sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2) / (kLightSpeed * kLightSpeed));
+  double v0 = u0 * ig, v1 = u1 * ig, v2 = velocity;
+  double gam = 1.0 / sqrt(1.0 - (v0 * v0 + v1 * v1 + v2 * v2) / (kLightSpeed * kLightSpeed));
+  cell[j] = pack(cell_i(c), k);
+  fb[j] = (float)(xz - k);
+  ur[j] = (float)(gam * v0);
+  uphi[j] = (float)(gam * v1);
+  uz[j] = (float)(gam * v2);
+}
+
+F32Args args32(Ctx &c, Species &sp) {
+  F32Args p;
+  const GridDesc &g = c.g;
+  p.nr = g.nr;
+  p.nz = g.nz;
+  p.hr = (float)(g.dt / (2.0 * g.dr));
+  p.hz = (float)(g.dt / (2.0 * g.dz));
+  p.inv_c2 = (float)(1.0 / (kLightSpeed * kLightSpeed));
+  double c1 = sp.charge * g.dt / (2.0 * sp.mass);
+  p.c1 = (float)c1;
+  if (c.pusher == 1) {
+    p.escale = (float)(2.0 * c1);
+    p.bscale = (float)(kMagnConst * c1 * 2.0 * c1 * c1);
+  } else {
+    p.escale = (float)c1;
+    p.bscale = (float)(kMagnConst * c1);
+  }
+  p.kz = (float)(sp.charge / (4.0 * kPi * g.dr * g.dr * g.dt));
+  p.kr = (float)(sp.charge / (2.0 * kPi * g.dr * g.dz * g.dt));
+  p.kp = (float)(sp.charge / (4.0 * kPi * g.dz * g.dr * g.dr));
+  p.pusher = c.pusher;
+  p.is_bunch = sp.kind == 1;
+  p.want_rho = sp.kind == 1;
+  p.n = sp.n;
+  p.er = c.a[kER];
+  p.ephi = c.a[kEPhi];
+  p.ez = c.a[kEZ];
+  p.hhr = c.a[kHHR];
+  p.hhphi = c.a[kHHPhi];
+  p.hhz = c.a[kHHZ];
+  p.jr = c.a[kJR];
+  p.jphi = c.a[kJPhi];
+  p.jz = c.a[kJZ];
+  p.rho = c.a[kRhoBunch];
+  p.err = c.err;
+  p.cell = (uint32_t *)sp.f32.cell;
+  p.fa = sp.f32.fr;
+  p.fb = sp.f32.fz;
+  p.ur = sp.f32.vr;
+  p.uphi = sp.f32.vphi;
+  p.uz = sp.f32.vz;
+  p.w = sp.f32.w;
+  return p;
+}
+
+void ensure_f32(Species &sp) {
+  if (sp.f32.cell) return;
+  size_t n = sp.n > 0 ? (size_t)sp.n : 1;
+  (void)hipMalloc((void **)&sp.f32.cell, n * sizeof(int32_t));
+  (void)hipMalloc((void **)&sp.f32.fr, n * sizeof(float));
+  (void)hipMalloc((void **)&sp.f32.fz, n * sizeof(float));
+  (void)hipMalloc((void **)&sp.f32.vr, n * sizeof(float));
+  (void)hipMalloc((void **)&sp.f32.vphi, n * sizeof(float));
+  (void)hipMalloc((void **)&sp.f32.vz, n * sizeof(float));
+  (void)hipMalloc((void **)&sp.f32.w, n * sizeof(float));
+}
+
+unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void launch_fused_f32(Ctx &c, Species &sp) {
+  if (sp.n <= 0 || !sp.f32.cell) return;
+  hipLaunchKernelGGL(k_fused32, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, args32(c, sp));
+}
+
+void launch_charge_weighting_f32(Ctx &c, Species &sp) {
+  if (sp.n <= 0 || !sp.f32.cell) return;
+  hipLaunchKernelGGL(k_charge32, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, args32(c, sp));
+}
+
+void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl) {
+  if (count == 0 || !sp.f32.cell) return;
+  hipLaunchKernelGGL(k_inject32, dim3((count + 255) / 256), dim3(256), 0, c.stream, c.g,
+                     (uint32_t *)sp.f32.cell, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.inject_u, sp.n,
+                     start, count, dl, sp.velocity);
+}
+
+void launch_f32_from_f64(Ctx &c, Species &sp) {
+  if (sp.n <= 0) return;
+  ensure_f32(sp);
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z, sp.f64.vr,
+                     sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, (uint32_t *)sp.f32.cell, sp.f32.fr,
+                     sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.f32.w);
+}
+
+void launch_f64_from_f32(Ctx &c, Species &sp) {
+  if (sp.n <= 0 || !sp.f32.cell) return;
+  hipLaunchKernelGGL(k_unpack, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z,
+                     sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, (const uint32_t *)sp.f32.cell,
+                     sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.f32.w);
+}
+
 }  // namespace pdp3
