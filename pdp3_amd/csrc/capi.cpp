@@ -88,24 +88,34 @@ void free_species(Species &sp) {
   sp.inject_u = nullptr;
 }
 
+bool is_f32(const Ctx &c) { return c.precision == PDP3_PREC_FP32; }
+
+// FP64 arrays: the live state in the species' layout (n_cap slots) in the
+// FP64 form, a logical-order staging copy (n entries) in the FP32 form
 int alloc_species_state(pdp3_ctx *ctx, Species &sp) {
-  const size_t n = (size_t)sp.n;
+  Ctx &c = ctx->c;
+  sp.n_cap = sp.chunked ? ((long long)sp.n + kChunk - 1) / kChunk * kChunk : sp.n;
+  const size_t n = is_f32(c) ? (size_t)sp.n : (size_t)sp.n_cap;
   int st;
   if ((st = dalloc(ctx, &sp.f64.r, n)) || (st = dalloc(ctx, &sp.f64.z, n)) || (st = dalloc(ctx, &sp.f64.vr, n)) ||
       (st = dalloc(ctx, &sp.f64.vphi, n)) || (st = dalloc(ctx, &sp.f64.vz, n)) ||
       (st = dalloc(ctx, &sp.f64.w, n)) || (st = dalloc(ctx, &sp.f64.alive, n)))
     return st;
-  HIP_TRY(ctx, hipMemsetAsync(sp.f64.alive, 0, n, ctx->c.stream));
+  HIP_TRY(ctx, hipMemsetAsync(sp.f64.alive, 0, n, c.stream));
   for (double *p : {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w})
-    HIP_TRY(ctx, hipMemsetAsync(p, 0, n * sizeof(double), ctx->c.stream));
+    HIP_TRY(ctx, hipMemsetAsync(p, 0, n * sizeof(double), c.stream));
+  if (!is_f32(c) && sp.chunked) {
+    launch_mark_padding(c, sp.f64.alive, sp.n, sp.n_cap);
+    CHECK_LAUNCH(ctx);
+  }
   return PDP3_OK;
 }
 
 int ensure_split_scratch(pdp3_ctx *ctx, Species &sp) {
   if (sp.f64.r_old) return PDP3_OK;
   int st;
-  if ((st = dalloc(ctx, &sp.f64.r_old, (size_t)sp.n)) || (st = dalloc(ctx, &sp.f64.z_old, (size_t)sp.n)) ||
-      (st = dalloc(ctx, &sp.f64.sin_t, (size_t)sp.n)))
+  if ((st = dalloc(ctx, &sp.f64.r_old, (size_t)sp.n_cap)) || (st = dalloc(ctx, &sp.f64.z_old, (size_t)sp.n_cap)) ||
+      (st = dalloc(ctx, &sp.f64.sin_t, (size_t)sp.n_cap)))
     return st;
   return PDP3_OK;
 }
@@ -163,8 +173,6 @@ int check_device_errors(pdp3_ctx *ctx) {
   }
   return PDP3_OK;
 }
-
-bool is_f32(const Ctx &c) { return c.precision == PDP3_PREC_FP32; }
 
 int inject_bunch(pdp3_ctx *ctx, Species &sp) {
   // Bunch::bunch_inject, src/bunch.cpp:87-124 (the index arithmetic runs here
@@ -245,6 +253,8 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
   if (pusher != PDP3_PUSHER_BORIS && pusher != PDP3_PUSHER_VAY) return PDP3_ERR_BAD_ARG;
   if (precision != PDP3_PREC_FP64 && precision != PDP3_PREC_FP32) return PDP3_ERR_BAD_ARG;
   if ((long long)n_grid_r * n_grid_z >= (1LL << 31)) return PDP3_ERR_BAD_ARG;
+  // the FP32 form packs the cell as 15 + 16 bits
+  if (precision == PDP3_PREC_FP32 && (n_grid_r > 32768 || n_grid_z > 65536)) return PDP3_ERR_BAD_ARG;
   pdp3_ctx *ctx = new (std::nothrow) pdp3_ctx();
   if (!ctx) return PDP3_ERR_OOM;
   Ctx &c = ctx->c;
@@ -548,8 +558,8 @@ int pdp3_species_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, int
   sp.mass = mass_e * kElMass;
   sp.n = number;
   sp.created = true;
+  sp.chunked = true;
   int st = alloc_species_state(ctx, sp);
-  if (!st && is_f32(c)) st = 0;  // the FP32 arrays are allocated at upload / init
   if (st) {
     free_species(sp);
     return st;
@@ -580,6 +590,7 @@ int pdp3_bunch_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, const
   sp.velocity = velocity;
   sp.bunch_number = bunch_number;
   sp.created = false;
+  sp.chunked = false;
   int st = alloc_species_state(ctx, sp);
   if (!st) st = dalloc(ctx, &sp.inject_u, (size_t)number);
   if (st) {
@@ -596,7 +607,7 @@ int pdp3_bunch_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number, const
   c.species.push_back(sp);
   int idx = (int)c.species.size() - 1;
   if (is_f32(c)) {
-    launch_f32_from_f64(c, c.species[idx]);
+    if (!launch_f32_from_f64(c, c.species[idx])) return fail(ctx, PDP3_ERR_OOM, "FP32 species arrays");
     CHECK_LAUNCH(ctx);
   }
   if (out_index) *out_index = idx;
@@ -616,19 +627,37 @@ int pdp3_species_upload(pdp3_ctx *ctx, int index, const double *r, const double 
   Species &sp = c.species[index];
   const size_t b = sizeof(double) * (size_t)sp.n;
   if (sp.n == 0) return PDP3_OK;
-  if (is_f32(c)) {
-    // keep the FP64 mirror coherent before overwriting parts of it
-    launch_f64_from_f32(c, sp);
-    CHECK_LAUNCH(ctx);
-  }
   const double *src[6] = {r, z, vr, vphi, vz, w};
   double *dst[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
-  for (int j = 0; j < 6; j++)
-    if (src[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyHostToDevice, c.stream));
-  if (alive) HIP_TRY(ctx, hipMemcpyAsync(sp.f64.alive, alive, (size_t)sp.n, hipMemcpyHostToDevice, c.stream));
   if (is_f32(c)) {
-    launch_f32_from_f64(c, sp);
+    // refresh the logical-order FP64 staging, overwrite what is given, repack
+    launch_f64_from_f32(c, sp);
     CHECK_LAUNCH(ctx);
+    for (int j = 0; j < 6; j++)
+      if (src[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyHostToDevice, c.stream));
+    if (alive) HIP_TRY(ctx, hipMemcpyAsync(sp.f64.alive, alive, (size_t)sp.n, hipMemcpyHostToDevice, c.stream));
+    if (!launch_f32_from_f64(c, sp)) return fail(ctx, PDP3_ERR_OOM, "FP32 species arrays");
+    CHECK_LAUNCH(ctx);
+  } else if (sp.chunked) {
+    void *tmp = nullptr;
+    HIP_TRY(ctx, hipMalloc(&tmp, b));
+    for (int j = 0; j < 6; j++)
+      if (src[j]) {
+        (void)hipMemcpyAsync(tmp, src[j], b, hipMemcpyHostToDevice, c.stream);
+        launch_to_phys(c, (const double *)tmp, dst[j], sp.n, sp.n_cap);
+      }
+    if (alive) {
+      (void)hipMemcpyAsync(tmp, alive, (size_t)sp.n, hipMemcpyHostToDevice, c.stream);
+      launch_alive_to_phys(c, (const uint8_t *)tmp, sp.f64.alive, sp.n, sp.n_cap);
+    }
+    hipError_t e = hipStreamSynchronize(c.stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(ctx, PDP3_ERR_HIP, hipGetErrorString(e));
+    CHECK_LAUNCH(ctx);
+  } else {
+    for (int j = 0; j < 6; j++)
+      if (src[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyHostToDevice, c.stream));
+    if (alive) HIP_TRY(ctx, hipMemcpyAsync(sp.f64.alive, alive, (size_t)sp.n, hipMemcpyHostToDevice, c.stream));
   }
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   return PDP3_OK;
@@ -641,12 +670,31 @@ int pdp3_species_download(pdp3_ctx *ctx, int index, double *r, double *z, double
   Species &sp = c.species[index];
   const size_t b = sizeof(double) * (size_t)sp.n;
   if (sp.n == 0) return PDP3_OK;
+  double *dst[6] = {r, z, vr, vphi, vz, w};
+  const double *src[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
+  if (!is_f32(c) && sp.chunked) {
+    void *tmp = nullptr;
+    HIP_TRY(ctx, hipMalloc(&tmp, b));
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < 6 && e == hipSuccess; j++)
+      if (dst[j]) {
+        launch_to_logical(c, src[j], (double *)tmp, sp.n);
+        e = hipMemcpyAsync(dst[j], tmp, b, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+      }
+    if (alive && e == hipSuccess) {
+      launch_alive_to_logical(c, sp.f64.alive, (uint8_t *)tmp, sp.n);
+      e = hipMemcpyAsync(alive, tmp, (size_t)sp.n, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(ctx, PDP3_ERR_HIP, hipGetErrorString(e));
+    return PDP3_OK;
+  }
   if (is_f32(c)) {
     launch_f64_from_f32(c, sp);
     CHECK_LAUNCH(ctx);
   }
-  double *dst[6] = {r, z, vr, vphi, vz, w};
-  const double *src[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
   for (int j = 0; j < 6; j++)
     if (dst[j]) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyDeviceToHost, c.stream));
   if (alive) HIP_TRY(ctx, hipMemcpyAsync(alive, sp.f64.alive, (size_t)sp.n, hipMemcpyDeviceToHost, c.stream));
@@ -663,10 +711,10 @@ int pdp3_species_init_uniform(pdp3_ctx *ctx, int index, uint64_t seed, int ppc, 
   long long want = (long long)ppc * (c.g.nr - 1) * (c.g.nz - 1);
   if (sp.kind != PDP3_SPECIES_PLASMA || want != sp.n)
     return fail(ctx, PDP3_ERR_BAD_ARG, "species size must be ppc * (n_grid_r-1) * (n_grid_z-1)");
-  launch_init_uniform(c, sp, index, seed, ppc, density, ramp_cells, temperature_ev);
+  launch_init_uniform(c, sp, index, seed, ppc, density, ramp_cells, temperature_ev, !is_f32(c) && sp.chunked);
   CHECK_LAUNCH(ctx);
   if (is_f32(c)) {
-    launch_f32_from_f64(c, sp);
+    if (!launch_f32_from_f64(c, sp)) return fail(ctx, PDP3_ERR_OOM, "FP32 species arrays");
     CHECK_LAUNCH(ctx);
   }
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));

@@ -37,6 +37,33 @@ struct GridDesc {
   double dt;
 };
 
+// Physical layout of plasma species.  Particles are processed by wavefronts
+// in chunks of 64 * kPPT: lane L of a chunk handles the kPPT consecutive
+// logical (cell-sorted) particles L*kPPT .. L*kPPT+kPPT-1, and iteration j of
+// all lanes reads the contiguous slots j*64 .. j*64+63, so loads stay
+// coalesced while every lane walks its own run of same-cell particles.
+// Species arrays hold n_cap = n rounded up to a whole number of chunks; the
+// extra slots are dead padding.  Bunches use the plain layout (slot =
+// logical index; their injection addresses particles by index).
+constexpr int kPPT = 16;
+constexpr int kChunk = 64 * kPPT;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlockParticles = kWavesPerBlock * kChunk;
+
+__host__ __device__ __forceinline__ long long phys_of(long long s) {
+  long long c = s / kChunk, w = s - c * kChunk;
+  return c * kChunk + (w % kPPT) * 64 + w / kPPT;
+}
+__host__ __device__ __forceinline__ long long logical_of(long long p) {
+  long long c = p / kChunk, w = p - c * kChunk;
+  return c * kChunk + (w % 64) * kPPT + w / 64;
+}
+
+// alive byte of the FP64 state: 0 dead, 1 alive, 2 padding slot
+constexpr uint8_t kPadSlot = 2;
+// FP32 cell word of a padding slot (a dead particle's word never equals it)
+constexpr uint32_t kPadCell = 0xFFFFFFFFu;
+
 // FP64 particle state of one species (structure of arrays)
 struct SpeciesF64 {
   double *r = nullptr, *z = nullptr, *vr = nullptr, *vphi = nullptr, *vz = nullptr, *w = nullptr;
@@ -57,7 +84,9 @@ struct Species {
   int charge_e = 0;
   double mass_e = 0;
   double charge = 0, mass = 0;  // per real particle, src/particles.cpp:30-33
-  int n = 0;
+  int n = 0;          // particles (logical count)
+  long long n_cap = 0;  // slots in the device arrays
+  bool chunked = false; // chunk-transposed layout (plasma) or plain (bunch)
   SpeciesF64 f64;
   SpeciesF32 f32;
   // sort scratch (second buffer of the same layout) and keys
@@ -124,12 +153,19 @@ void launch_bunch_inject_f64(Ctx &c, Species &sp, unsigned start, unsigned count
 void launch_fused_f32(Ctx &c, Species &sp);
 void launch_charge_weighting_f32(Ctx &c, Species &sp);
 void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
-void launch_f32_from_f64(Ctx &c, Species &sp);  // pack absolute FP64 state into cell form
+bool launch_f32_from_f64(Ctx &c, Species &sp);  // pack absolute FP64 state into cell form
 void launch_f64_from_f32(Ctx &c, Species &sp);  // unpack for download
 
-// synthetic loader
+// synthetic loader (writes the FP64 arrays; to_phys: in the chunked layout)
 void launch_init_uniform(Ctx &c, Species &sp, int species_index, uint64_t seed, int ppc,
-                         double density, int ramp_cells, double temperature_ev);
+                         double density, int ramp_cells, double temperature_ev, bool to_phys);
+// logical <-> chunked-layout copies of FP64 / byte arrays; padding slots of
+// `alive` are marked kPadSlot
+void launch_mark_padding(Ctx &c, uint8_t *alive, long long n, long long n_cap);
+void launch_to_phys(Ctx &c, const double *logical, double *phys, long long n, long long n_cap);
+void launch_to_logical(Ctx &c, const double *phys, double *logical, long long n);
+void launch_alive_to_phys(Ctx &c, const uint8_t *logical, uint8_t *phys, long long n, long long n_cap);
+void launch_alive_to_logical(Ctx &c, const uint8_t *phys, uint8_t *logical, long long n);
 
 // sort
 int sort_species(Ctx &c, Species &sp);

@@ -41,9 +41,10 @@ __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
 
 __global__ void k_init_uniform(GridDesc g, double *r, double *z, double *vr, double *vphi, double *vz, double *w,
                                uint8_t *alive, long long n, int ppc, uint32_t seed_lo, uint32_t seed_hi,
-                               uint32_t species, double density, int ramp_cells, double sigma_v) {
+                               uint32_t species, double density, int ramp_cells, double sigma_v, int to_phys) {
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const long long dst = to_phys ? phys_of(p) : p;
   const int ncz = g.nz - 1;
   long long cell = p / ppc;
   uint32_t slot = (uint32_t)(p - cell * ppc);
@@ -69,27 +70,81 @@ __global__ void k_init_uniform(GridDesc g, double *r, double *z, double *vr, dou
     double zr = zz / (ramp_cells * dz);
     if (zr < 1.0) dens = density * zr;
   }
-  r[p] = rr;
-  z[p] = zz;
-  vr[p] = sigma_v * n0;
-  vphi[p] = sigma_v * n1;
-  vz[p] = sigma_v * n2;
+  r[dst] = rr;
+  z[dst] = zz;
+  vr[dst] = sigma_v * n0;
+  vphi[dst] = sigma_v * n1;
+  vz[dst] = sigma_v * n2;
   // real particles represented: n(z) * 2 pi r (r_hi - r_lo)(z_hi - z_lo) / ppc
-  w[p] = dens * 2.0 * kPi * rr * (r_hi - r_lo) * (z_hi - z_lo) / ppc;
-  alive[p] = 1;
+  w[dst] = dens * 2.0 * kPi * rr * (r_hi - r_lo) * (z_hi - z_lo) / ppc;
+  alive[dst] = 1;
+}
+
+__global__ void k_mark_padding(uint8_t *alive, long long n, long long n_cap) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_cap) return;
+  if (logical_of(p) >= n) alive[p] = kPadSlot;
+}
+
+__global__ void k_to_phys(const double *src, double *dst, long long n, long long n_cap) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_cap) return;
+  long long s = logical_of(p);
+  dst[p] = s < n ? src[s] : 0.0;
+}
+__global__ void k_to_logical(const double *src, double *dst, long long n) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  dst[s] = src[phys_of(s)];
+}
+__global__ void k_alive_to_phys(const uint8_t *src, uint8_t *dst, long long n, long long n_cap) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_cap) return;
+  long long s = logical_of(p);
+  dst[p] = s < n ? src[s] : kPadSlot;
+}
+__global__ void k_alive_to_logical(const uint8_t *src, uint8_t *dst, long long n) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  dst[s] = src[phys_of(s)];
 }
 
 }  // namespace
 
+void launch_mark_padding(Ctx &c, uint8_t *alive, long long n, long long n_cap) {
+  if (n_cap > n)
+    hipLaunchKernelGGL(k_mark_padding, dim3((unsigned)((n_cap + 255) / 256)), dim3(256), 0, c.stream, alive, n,
+                       n_cap);
+}
+void launch_to_phys(Ctx &c, const double *logical, double *phys, long long n, long long n_cap) {
+  if (n_cap > 0)
+    hipLaunchKernelGGL(k_to_phys, dim3((unsigned)((n_cap + 255) / 256)), dim3(256), 0, c.stream, logical, phys, n,
+                       n_cap);
+}
+void launch_to_logical(Ctx &c, const double *phys, double *logical, long long n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_to_logical, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, phys, logical, n);
+}
+void launch_alive_to_phys(Ctx &c, const uint8_t *logical, uint8_t *phys, long long n, long long n_cap) {
+  if (n_cap > 0)
+    hipLaunchKernelGGL(k_alive_to_phys, dim3((unsigned)((n_cap + 255) / 256)), dim3(256), 0, c.stream, logical,
+                       phys, n, n_cap);
+}
+void launch_alive_to_logical(Ctx &c, const uint8_t *phys, uint8_t *logical, long long n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_alive_to_logical, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, phys,
+                       logical, n);
+}
+
 void launch_init_uniform(Ctx &c, Species &sp, int species_index, uint64_t seed, int ppc, double density,
-                         int ramp_cells, double temperature_ev) {
+                         int ramp_cells, double temperature_ev, bool to_phys) {
   long long n = sp.n;
   if (n <= 0) return;
   double sigma_v = sqrt(temperature_ev * kElCharge / sp.mass);
   unsigned blocks = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_init_uniform, dim3(blocks), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z, sp.f64.vr,
                      sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, n, ppc, (uint32_t)seed,
-                     (uint32_t)(seed >> 32), (uint32_t)species_index, density, ramp_cells, sigma_v);
+                     (uint32_t)(seed >> 32), (uint32_t)species_index, density, ramp_cells, sigma_v, to_phys ? 1 : 0);
 }
 
 }  // namespace pdp3

@@ -16,6 +16,7 @@
 // is evaluated in FP64; the grid accumulation is FP64 (hardware FP64
 // atomics); fields and the PML stay FP64.
 #include "context.hpp"
+#include "sink.hpp"
 
 namespace pdp3 {
 
@@ -40,19 +41,11 @@ struct F32Args {
   int want_rho;
   int n;
   const double *er, *ephi, *ez, *hhr, *hhphi, *hhz;
-  double *jr, *jphi, *jz, *rho;
   int *err;
   uint32_t *cell;
   float *fa, *fb, *ur, *uphi, *uz;
   const float *w;
 };
-
-__device__ __forceinline__ void add_node(const F32Args &p, double *a, int i, int k, double v) {
-  if ((unsigned)i < (unsigned)p.nr && (unsigned)k < (unsigned)p.nz)
-    unsafeAtomicAdd(a + (size_t)i * (size_t)p.nz + (size_t)k, v);
-  else
-    atomicOr(p.err, kErrDomain);
-}
 
 // radial weights of the four-node gathers (src/eField.cpp:204-316,
 // src/hField.cpp:159-267) in cell units.  "lo" family (E_r, H_z, H_phi):
@@ -192,7 +185,8 @@ __device__ __forceinline__ float rotate_back(int i, float &xa, float dy) {
 
 // J_phi of one particle at the half-step position,
 // azimuthal_current_distribution (src/particles.cpp:688-784) in cell units
-__device__ void deposit_jphi(const F32Args &p, float w, int i, float a, int k, float b, float vphi) {
+template <class S>
+__device__ void deposit_jphi(const F32Args &p, S &sk, float w, int i, float a, int k, float b, float vphi) {
   double c0, c1;  // node (i,k) and (i+1,k) weights before the axial split
   if (i == 0) {
     double ad = a;  // axis cell in FP64; rho = a here
@@ -204,14 +198,15 @@ __device__ void deposit_jphi(const F32Args &p, float w, int i, float a, int k, f
     c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
   }
   float s = p.kp * w * vphi;
-  add_node(p, p.jphi, i, k, (double)(s * (1.f - b)) * c0);
-  add_node(p, p.jphi, i + 1, k, (double)(s * (1.f - b)) * c1);
-  add_node(p, p.jphi, i, k + 1, (double)(s * b) * c0);
-  add_node(p, p.jphi, i + 1, k + 1, (double)(s * b) * c1);
+  sk.template add<kCPhi>(i, k, (double)(s * (1.f - b)) * c0);
+  sk.template add<kCPhi>(i + 1, k, (double)(s * (1.f - b)) * c1);
+  sk.template add<kCPhi>(i, k + 1, (double)(s * b) * c0);
+  sk.template add<kCPhi>(i + 1, k + 1, (double)(s * b) * c1);
 }
 
 // Particles::charge_weighting of one particle (src/particles.cpp:101-226)
-__device__ void deposit_rho(const F32Args &p, float w, int i, float a, int k, float b) {
+template <class S>
+__device__ void deposit_rho(const F32Args &p, S &sk, float w, int i, float a, int k, float b) {
   double c0, c1;
   if (i == 0) {
     double rho = a;
@@ -228,10 +223,10 @@ __device__ void deposit_rho(const F32Args &p, float w, int i, float a, int k, fl
     c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
   }
   float s = p.kp * w;
-  add_node(p, p.rho, i, k, (double)(s * (1.f - b)) * c0);
-  add_node(p, p.rho, i + 1, k, (double)(s * (1.f - b)) * c1);
-  add_node(p, p.rho, i, k + 1, (double)(s * b) * c0);
-  add_node(p, p.rho, i + 1, k + 1, (double)(s * b) * c1);
+  sk.template add<kCRho>(i, k, (double)(s * (1.f - b)) * c0);
+  sk.template add<kCRho>(i + 1, k, (double)(s * (1.f - b)) * c1);
+  sk.template add<kCRho>(i, k + 1, (double)(s * b) * c0);
+  sk.template add<kCRho>(i + 1, k + 1, (double)(s * b) * c1);
 }
 
 // (log(1+x) - x) / x without cancellation
@@ -252,7 +247,8 @@ __device__ __forceinline__ float m_of(float x) {
 // algebraically identical to the reference's expressions (which evaluate them
 // as differences of O(i) terms); they also reduce to strict_motion_weighting
 // (:786-988) for purely axial or radial motion.
-__device__ void segment(const F32Args &p, float w, int i, int k, float a, float b, float drho, float dzeta) {
+template <class S>
+__device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a, float b, float drho, float dzeta) {
   float jz0, jz1, jr0, jr1;
   if (i == 0) {
     double rs = a;  // axis cell in FP64
@@ -275,17 +271,18 @@ __device__ void segment(const F32Args &p, float w, int i, int k, float a, float 
     jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b / re - dzeta * N)) * inv;
   }
   float sz = p.kz * w * dzeta, sr = p.kr * w;
-  add_node(p, p.jz, i, k, (double)(sz * jz0));
-  add_node(p, p.jz, i + 1, k, (double)(sz * jz1));
-  add_node(p, p.jr, i, k, (double)(sr * jr0));
-  add_node(p, p.jr, i, k + 1, (double)(sr * jr1));
+  sk.template add<kCZ>(i, k, (double)(sz * jz0));
+  sk.template add<kCZ>(i + 1, k, (double)(sz * jz1));
+  sk.template add<kCR>(i, k, (double)(sr * jr0));
+  sk.template add<kCR>(i, k + 1, (double)(sr * jr1));
 }
 
 // current_distribution (src/particles.cpp:476-686): the straight path from
 // the old to the new position, split at the cell faces it crosses (at most
 // one radial and one axial face: |displacement| <= c dt <= dr/2), each piece
 // deposited in its own cell
-__device__ void deposit_j(const F32Args &p, float w, int io, float ao, int ko, float bo, int in_, float an,
+template <class S>
+__device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, int ko, float bo, int in_, float an,
                           int kn, float bn) {
   int di = in_ - io, dk = kn - ko;
   if (di < -1 || di > 1 || dk < -1 || dk > 1) {
@@ -309,14 +306,14 @@ __device__ void deposit_j(const F32Args &p, float w, int io, float ao, int ko, f
     float sd_r = (t - tprev) * drho, sd_z = (t - tprev) * dzeta;
     if (rfirst) {
       float fr = di > 0 ? 1.f : 0.f;
-      segment(p, w, ci, ck, ca, cb, fr - ca, sd_z);
+      segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z);
       ci += di;
       ca = di > 0 ? 0.f : 1.f;
       cb += sd_z;
       tr = 2.f;
     } else {
       float fz = dk > 0 ? 1.f : 0.f;
-      segment(p, w, ci, ck, ca, cb, sd_r, fz - cb);
+      segment(p, sk, w, ci, ck, ca, cb, sd_r, fz - cb);
       ck += dk;
       cb = dk > 0 ? 0.f : 1.f;
       ca += sd_r;
@@ -325,7 +322,7 @@ __device__ void deposit_j(const F32Args &p, float w, int io, float ao, int ko, f
     tprev = t;
   }
   // last piece to the new position
-  segment(p, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb);
+  segment(p, sk, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb);
 }
 
 // wall handling of move_half_reflect on the Cartesian half-step point
@@ -372,15 +369,16 @@ __device__ __forceinline__ void walls(const F32Args &p, int &i, float &xa, int &
   }
 }
 
-__global__ void __launch_bounds__(256) k_fused32(F32Args p) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= p.n) return;
+// the whole step of one particle in slot n
+template <class S>
+__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, long long n) {
   uint32_t c = p.cell[n];
   if (c & kDead) return;
   int i = cell_i(c), k = cell_k(c);
   float a = p.fa[n], b = p.fb[n];
   float u[3] = {p.ur[n], p.uphi[n], p.uz[n]};
   const float w = p.w[n];
+  sk.begin_cell(i, k);
   // gathers (E at the full step, H at the half-time copy)
   RadW rl = rad_lo(i, a), ro = rad_own(i, a);
   AxW zo = ax_own(k, b), zh = ax_half(k, b);
@@ -403,7 +401,7 @@ __global__ void __launch_bounds__(256) k_fused32(F32Args p) {
   rotate_back(i1, xa, dy);
   renorm(i1, xa);
   renorm(k1, b1);
-  deposit_jphi(p, w, i1, xa, k1, b1, u[1] * g);
+  deposit_jphi(p, sk, w, i1, xa, k1, b1, u[1] * g);
   // second half step with the walls (move_half_reflect, :1355-1365)
   float xa2 = xa + dx, b2 = b1 + dz;
   bool alive = true;
@@ -412,14 +410,14 @@ __global__ void __launch_bounds__(256) k_fused32(F32Args p) {
   renorm(i1, xa2);
   renorm(k1, b2);
   if (alive) {
-    deposit_j(p, w, i, a, k, b, i1, xa2, k1, b2);
+    deposit_j(p, sk, w, i, a, k, b, i1, xa2, k1, b2);
     // back_velocity_to_rz with the second half step's angle (:1005-1023)
     float ct = sqrtf(1.f - st * st);
     float u0 = ct * u[0] - st * u[1];
     float u1 = st * u[0] + ct * u[1];
     u[0] = u0;
     u[1] = u1;
-    if (p.want_rho) deposit_rho(p, w, i1, xa2, k1, b2);
+    if (p.want_rho) deposit_rho(p, sk, w, i1, xa2, k1, b2);
   }
   p.cell[n] = pack(i1, k1) | (alive ? 0u : kDead);
   p.fa[n] = xa2;
@@ -429,20 +427,64 @@ __global__ void __launch_bounds__(256) k_fused32(F32Args p) {
   p.uz[n] = u[2];
 }
 
-__global__ void k_charge32(F32Args p) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= p.n) return;
-  uint32_t c = p.cell[n];
-  if (c & kDead) return;
-  deposit_rho(p, p.w[n], cell_i(c), p.fa[n], cell_k(c), p.fb[n]);
+// chunked plasma species: 4 wavefronts per block, each walking one chunk of
+// kChunk slots kPPT particles per lane, deposits through the tile sink
+__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
+  __shared__ double box[kBoxSize];
+  const long long base = (long long)blockIdx.x * kBlockParticles;
+  int ai = 0, ak = 0;
+  if (base < n_cap) {
+    uint32_t c0 = p.cell[base];
+    if (!(c0 & kDead)) {
+      ai = cell_i(c0);
+      ak = cell_k(c0);
+    }
+  }
+  TileSink sk;
+  box_init(box, ai, ak, sk, gs);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long cbase = base + (long long)wave * kChunk + lane;
+  for (int j = 0; j < kPPT; j++) {
+    long long n = cbase + (long long)j * 64;
+    if (n < n_cap) step_particle32(p, sk, n);
+  }
+  sk.flush();
+  __syncthreads();
+  box_flush(box, sk);
 }
 
-// absolute FP64 state -> cell form (cell index by CELL_NUMBER clamped at 0)
+// plain layout (bunches): one particle per thread, deposits straight to HBM
+__global__ void __launch_bounds__(256) k_fused32_plain(F32Args p, GlobalSink gs, long long n_cap) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_cap) return;
+  GlobalSink sk = gs;
+  step_particle32(p, sk, n);
+}
+
+__global__ void k_charge32(F32Args p, GlobalSink gs, long long n_cap) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_cap) return;
+  uint32_t c = p.cell[n];
+  if (c & kDead) return;
+  GlobalSink sk = gs;
+  deposit_rho(p, sk, p.w[n], cell_i(c), p.fa[n], cell_k(c), p.fb[n]);
+}
+
+// absolute FP64 state in logical order -> cell form in the species' layout
+// (cell index by CELL_NUMBER clamped at 0; chunk padding slots marked)
 __global__ void k_pack(GridDesc g, const double *r, const double *z, const double *vr, const double *vphi,
-                       const double *vz, const double *w, const uint8_t *alive, int n, uint32_t *cell, float *fa,
-                       float *fb, float *ur, float *uphi, float *uz, float *fw) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+                       const double *vz, const double *w, const uint8_t *alive, int n, long long n_cap, int chunked,
+                       uint32_t *cell, float *fa, float *fb, float *ur, float *uphi, float *uz, float *fw) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_cap) return;
+  const long long j = chunked ? logical_of(p) : p;
+  if (j >= n) {
+    cell[p] = kPadCell;
+    fa[p] = fb[p] = 0.5f;
+    ur[p] = uphi[p] = uz[p] = fw[p] = 0.f;
+    return;
+  }
   double xr = r[j] / g.dr, xz = z[j] / g.dz;
   int i = (int)ceil(xr) - 1, k = (int)ceil(xz) - 1;
   if (i < 0) i = 0;
@@ -452,29 +494,30 @@ __global__ void k_pack(GridDesc g, const double *r, const double *z, const doubl
   float oa = (float)(xr - i), ob = (float)(xz - k);
   double v2 = vr[j] * vr[j] + vphi[j] * vphi[j] + vz[j] * vz[j];
   double gam = 1.0 / sqrt(1.0 - v2 / (kLightSpeed * kLightSpeed));
-  cell[j] = pack(i, k) | (alive[j] ? 0u : kDead);
-  fa[j] = oa;
-  fb[j] = ob;
-  ur[j] = (float)(gam * vr[j]);
-  uphi[j] = (float)(gam * vphi[j]);
-  uz[j] = (float)(gam * vz[j]);
-  fw[j] = (float)w[j];
+  cell[p] = pack(i, k) | (alive[j] == 1 ? 0u : kDead);
+  fa[p] = oa;
+  fb[p] = ob;
+  ur[p] = (float)(gam * vr[j]);
+  uphi[p] = (float)(gam * vphi[j]);
+  uz[p] = (float)(gam * vz[j]);
+  fw[p] = (float)w[j];
 }
 
 __global__ void k_unpack(GridDesc g, double *r, double *z, double *vr, double *vphi, double *vz, double *w,
-                         uint8_t *alive, int n, const uint32_t *cell, const float *fa, const float *fb,
+                         uint8_t *alive, int n, int chunked, const uint32_t *cell, const float *fa, const float *fb,
                          const float *ur, const float *uphi, const float *uz, const float *fw) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  uint32_t c = cell[j];
-  r[j] = ((double)cell_i(c) + (double)fa[j]) * g.dr;
-  z[j] = ((double)cell_k(c) + (double)fb[j]) * g.dz;
-  double u0 = ur[j], u1 = uphi[j], u2 = uz[j];
+  const long long p = chunked ? phys_of(j) : j;
+  uint32_t c = cell[p];
+  r[j] = ((double)cell_i(c) + (double)fa[p]) * g.dr;
+  z[j] = ((double)cell_k(c) + (double)fb[p]) * g.dz;
+  double u0 = ur[p], u1 = uphi[p], u2 = uz[p];
   double ig = 1.0 / sqrt(1.0 + (u0 * u0 + u1 * u1 + u2 * u2) / (kLightSpeed * kLightSpeed));
   vr[j] = u0 * ig;
   vphi[j] = u1 * ig;
   vz[j] = u2 * ig;
-  w[j] = fw[j];
+  w[j] = fw[p];
   alive[j] = (c & kDead) ? 0 : 1;
 }
 
@@ -532,10 +575,6 @@ F32Args args32(Ctx &c, Species &sp) {
   p.hhr = c.a[kHHR];
   p.hhphi = c.a[kHHPhi];
   p.hhz = c.a[kHHZ];
-  p.jr = c.a[kJR];
-  p.jphi = c.a[kJPhi];
-  p.jz = c.a[kJZ];
-  p.rho = c.a[kRhoBunch];
   p.err = c.err;
   p.cell = (uint32_t *)sp.f32.cell;
   p.fa = sp.f32.fr;
@@ -547,30 +586,50 @@ F32Args args32(Ctx &c, Species &sp) {
   return p;
 }
 
-void ensure_f32(Species &sp) {
-  if (sp.f32.cell) return;
-  size_t n = sp.n > 0 ? (size_t)sp.n : 1;
-  (void)hipMalloc((void **)&sp.f32.cell, n * sizeof(int32_t));
-  (void)hipMalloc((void **)&sp.f32.fr, n * sizeof(float));
-  (void)hipMalloc((void **)&sp.f32.fz, n * sizeof(float));
-  (void)hipMalloc((void **)&sp.f32.vr, n * sizeof(float));
-  (void)hipMalloc((void **)&sp.f32.vphi, n * sizeof(float));
-  (void)hipMalloc((void **)&sp.f32.vz, n * sizeof(float));
-  (void)hipMalloc((void **)&sp.f32.w, n * sizeof(float));
+GlobalSink sink32(Ctx &c) {
+  GlobalSink gs;
+  gs.nr = c.g.nr;
+  gs.nz = c.g.nz;
+  gs.jr = c.a[kJR];
+  gs.jphi = c.a[kJPhi];
+  gs.jz = c.a[kJZ];
+  gs.rho = c.a[kRhoBunch];
+  gs.err = c.err;
+  return gs;
 }
 
-unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
+bool ensure_f32(Species &sp) {
+  if (sp.f32.cell) return true;
+  size_t n = sp.n_cap > 0 ? (size_t)sp.n_cap : 1;
+  bool ok = hipMalloc((void **)&sp.f32.cell, n * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.fr, n * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.fz, n * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.vr, n * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.vphi, n * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.vz, n * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&sp.f32.w, n * sizeof(float)) == hipSuccess;
+  return ok;
+}
+
+unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
 void launch_fused_f32(Ctx &c, Species &sp) {
-  if (sp.n <= 0 || !sp.f32.cell) return;
-  hipLaunchKernelGGL(k_fused32, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, args32(c, sp));
+  if (sp.n_cap <= 0 || !sp.f32.cell) return;
+  if (sp.chunked) {
+    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
+    hipLaunchKernelGGL(k_fused32_tiled, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);
+  } else {
+    hipLaunchKernelGGL(k_fused32_plain, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, args32(c, sp),
+                       sink32(c), sp.n_cap);
+  }
 }
 
 void launch_charge_weighting_f32(Ctx &c, Species &sp) {
-  if (sp.n <= 0 || !sp.f32.cell) return;
-  hipLaunchKernelGGL(k_charge32, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, args32(c, sp));
+  if (sp.n_cap <= 0 || !sp.f32.cell) return;
+  hipLaunchKernelGGL(k_charge32, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, args32(c, sp), sink32(c),
+                     sp.n_cap);
 }
 
 void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl) {
@@ -580,19 +639,21 @@ void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count
                      start, count, dl, sp.velocity);
 }
 
-void launch_f32_from_f64(Ctx &c, Species &sp) {
-  if (sp.n <= 0) return;
-  ensure_f32(sp);
-  hipLaunchKernelGGL(k_pack, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z, sp.f64.vr,
-                     sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, (uint32_t *)sp.f32.cell, sp.f32.fr,
-                     sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.f32.w);
+bool launch_f32_from_f64(Ctx &c, Species &sp) {
+  if (sp.n_cap <= 0) return true;
+  if (!ensure_f32(sp)) return false;
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z, sp.f64.vr,
+                     sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, sp.n_cap, sp.chunked ? 1 : 0,
+                     (uint32_t *)sp.f32.cell, sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.f32.w);
+  return true;
 }
 
 void launch_f64_from_f32(Ctx &c, Species &sp) {
   if (sp.n <= 0 || !sp.f32.cell) return;
   hipLaunchKernelGGL(k_unpack, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z,
-                     sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, (const uint32_t *)sp.f32.cell,
-                     sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz, sp.f32.w);
+                     sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, sp.chunked ? 1 : 0,
+                     (const uint32_t *)sp.f32.cell, sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz,
+                     sp.f32.w);
 }
 
 }  // namespace pdp3

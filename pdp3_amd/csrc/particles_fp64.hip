@@ -7,6 +7,7 @@
 // differs (hardware FP64 atomics instead of the reference's serial loop).
 #include "context.hpp"
 #include "crmath.hpp"
+#include "sink.hpp"
 
 namespace pdp3 {
 
@@ -14,11 +15,6 @@ namespace {
 
 struct Fields {
   const double *er, *ephi, *ez, *hhr, *hhphi, *hhz;
-};
-
-struct Accum {
-  double *jr, *jphi, *jz, *rho;
-  int *err;
 };
 
 struct SpeciesArgs {
@@ -41,13 +37,6 @@ __device__ __forceinline__ double cyl_vol(double z, double r) { return kPi * z *
 // `F * CYL_RNG_VOL(dzx, ra, rb) / vol` as the reference's macro expands
 __device__ __forceinline__ double ring_term(double F, double dzx, double ra, double rb, double vol) {
   return F * kPi * dzx * (rb * rb - ra * ra) / vol;
-}
-
-__device__ __forceinline__ void add_node(const GridDesc &g, double *a, int i, int k, double v, int *err) {
-  if ((unsigned)i < (unsigned)g.nr && (unsigned)k < (unsigned)g.nz)
-    unsafeAtomicAdd(a + (size_t)i * (size_t)g.nz + (size_t)k, v);
-  else
-    atomicOr(err, kErrDomain);
 }
 
 // four-node ring-volume gather shared by EField::get_field
@@ -195,7 +184,8 @@ __device__ bool vay(double q, double m, double dt, double e[3], double b[3], dou
 }
 
 // azimuthal_current_distribution for one particle, src/particles.cpp:688-784
-__device__ void deposit_jphi(const GridDesc &g, const Accum &acc, double q, double r, double z, double vphi) {
+template <class S>
+__device__ void deposit_jphi(const GridDesc &g, S &sk, double q, double r, double z, double vphi) {
   const double dr = g.dr, dz = g.dz;
   int r_i = cell_number(r, dr);
   int z_k = cell_number(z, dz);
@@ -209,14 +199,15 @@ __device__ void deposit_jphi(const GridDesc &g, const Accum &acc, double q, doub
   double v_2 = cell_volume(r_i + 1, dr, dz);
   double dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
   double dz2 = (z + 0.5 * dz) - (z_k + 0.5) * dz;
-  add_node(g, acc.jphi, r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1) * vphi, acc.err);
-  add_node(g, acc.jphi, r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2) * vphi, acc.err);
-  add_node(g, acc.jphi, r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1) * vphi, acc.err);
-  add_node(g, acc.jphi, r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2) * vphi, acc.err);
+  sk.template add<kCPhi>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1) * vphi);
+  sk.template add<kCPhi>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2) * vphi);
+  sk.template add<kCPhi>(r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1) * vphi);
+  sk.template add<kCPhi>(r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2) * vphi);
 }
 
 // Particles::charge_weighting for one particle, src/particles.cpp:101-226
-__device__ void deposit_rho(const GridDesc &g, double *rho, int *err, double q, double r, double z) {
+template <class S>
+__device__ void deposit_rho(const GridDesc &g, S &sk, double q, double r, double z) {
   const double dr = g.dr, dz = g.dz;
   int r_i = cell_number(r, dr);
   int z_k = cell_number(z, dz);
@@ -228,27 +219,27 @@ __device__ void deposit_rho(const GridDesc &g, double *rho, int *err, double q, 
     double r1 = r - 0.5 * dr, r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (2. * kPi * dz * dr * r);
     double v_1 = cell_volume(r_i, dr, dz), v_2 = cell_volume(r_i + 1, dr, dz);
-    add_node(g, rho, r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1), err);
-    add_node(g, rho, r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2), err);
-    add_node(g, rho, r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1), err);
-    add_node(g, rho, r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2), err);
+    sk.template add<kCRho>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1));
+    sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
+    sk.template add<kCRho>(r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1));
+    sk.template add<kCRho>(r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2));
   } else if (r <= dr / 2.) {
     r_i = 0;
     double r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (kPi * dz * (2. * r * r + dr * dr / 2.));
     double v_1 = cyl_vol(dz, dr), v_2 = cell_volume(r_i + 1, dr, dz);
-    add_node(g, rho, r_i, z_k, ro_v * kPi * dz1 * (dr * dr / 2. - r * dr + r * r) / v_1, err);
-    add_node(g, rho, r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2), err);
-    add_node(g, rho, r_i, z_k + 1, ro_v * kPi * dz2 * (dr * dr / 2. - r * dr + r * r) / v_1, err);
-    add_node(g, rho, r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2), err);
+    sk.template add<kCRho>(r_i, z_k, ro_v * kPi * dz1 * (dr * dr / 2. - r * dr + r * r) / v_1);
+    sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
+    sk.template add<kCRho>(r_i, z_k + 1, ro_v * kPi * dz2 * (dr * dr / 2. - r * dr + r * r) / v_1);
+    sk.template add<kCRho>(r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2));
   } else {
     double r1 = r - 0.5 * dr, r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (2. * kPi * dz * dr * r);
     double v_1 = cyl_vol(dz, dr), v_2 = cell_volume(r_i + 1, dr, dz);
-    add_node(g, rho, r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1), err);
-    add_node(g, rho, r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2), err);
-    add_node(g, rho, r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1), err);
-    add_node(g, rho, r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2), err);
+    sk.template add<kCRho>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1));
+    sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
+    sk.template add<kCRho>(r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1));
+    sk.template add<kCRho>(r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2));
   }
 }
 
@@ -265,7 +256,8 @@ __device__ __forceinline__ double density_z(double q, double r, double dr, doubl
 
 // simple_current_distribution: one straight segment inside cell (i_n, k_n),
 // src/particles.cpp:333-474
-__device__ void segment(const GridDesc &g, const Accum &acc, double q, double radius_new, double longitude_new,
+template <class S>
+__device__ void segment(const GridDesc &g, S &acc, double q, double radius_new, double longitude_new,
                         double radius_old, double longitude_old, int i_n, int k_n) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
   double delta_r = radius_new - radius_old;
@@ -288,8 +280,8 @@ __device__ void segment(const GridDesc &g, const Accum &acc, double q, double ra
     ssd = density_r(q, dr, dr, dz, delta_t);
     wj1 = ssd * (k * delta_z * delta_z / 2. + delta_z * dr + delta_z * b);
   }
-  add_node(g, acc.jz, i_n, k_n, wj0, acc.err);
-  add_node(g, acc.jz, i_n + 1, k_n, wj1, acc.err);
+  acc.template add<kCZ>(i_n, k_n, wj0);
+  acc.template add<kCZ>(i_n + 1, k_n, wj1);
   // J_r (identical in both radial branches)
   k = -delta_z / delta_r;
   double r0 = (i_n + 0.5) * dr;
@@ -301,17 +293,18 @@ __device__ void segment(const GridDesc &g, const Accum &acc, double q, double ra
                      0.5 * delta_r * (b - k * (2 * r0 + r1)) +
                      delta_r * (b - k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old * (radius_old + delta_r)) +
                      (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
-  add_node(g, acc.jr, i_n, k_n, wj, acc.err);
+  acc.template add<kCR>(i_n, k_n, wj);
   b = longitude_old - k_n * dz;
   wj = ssd * (-r0 * k * delta_r - k / 2. * delta_r * (radius_old + delta_r / 2.) +
               0.5 * delta_r * (b + k * (2 * r0 + r1)) +
               delta_r * (b + k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old * (radius_old + delta_r)) -
               (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
-  add_node(g, acc.jr, i_n, k_n + 1, wj, acc.err);
+  acc.template add<kCR>(i_n, k_n + 1, wj);
 }
 
 // strict_motion_weighting, src/particles.cpp:786-988
-__device__ void strict_motion(const GridDesc &g, const Accum &acc, double q, double radius_new,
+template <class S>
+__device__ void strict_motion(const GridDesc &g, S &acc, double q, double radius_new,
                               double longitude_new, double radius_old, double longitude_old) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
   int i_n = cell_number(radius_new, dr);
@@ -339,22 +332,22 @@ __device__ void strict_motion(const GridDesc &g, const Accum &acc, double q, dou
     int res_k = k_n - k_o;
     if (res_k == 0) {
       double delta_z = longitude_new - longitude_old;
-      add_node(g, acc.jz, i_n, k_n, wl * delta_z, acc.err);
-      add_node(g, acc.jz, i_n + 1, k_n, wu * delta_z, acc.err);
+      acc.template add<kCZ>(i_n, k_n, wl * delta_z);
+      acc.template add<kCZ>(i_n + 1, k_n, wu * delta_z);
     } else if (res_k == 1) {
       double delta_z = k_n * dz - longitude_old;
-      add_node(g, acc.jz, i_n, k_n - 1, wl * delta_z, acc.err);
-      add_node(g, acc.jz, i_n + 1, k_n - 1, wu * delta_z, acc.err);
+      acc.template add<kCZ>(i_n, k_n - 1, wl * delta_z);
+      acc.template add<kCZ>(i_n + 1, k_n - 1, wu * delta_z);
       delta_z = longitude_new - k_n * dz;
-      add_node(g, acc.jz, i_n, k_n, wl * delta_z, acc.err);
-      add_node(g, acc.jz, i_n + 1, k_n, wu * delta_z, acc.err);
+      acc.template add<kCZ>(i_n, k_n, wl * delta_z);
+      acc.template add<kCZ>(i_n + 1, k_n, wu * delta_z);
     } else if (res_k == -1) {
       double delta_z = (k_n + 1) * dz - longitude_old;
-      add_node(g, acc.jz, i_n, k_n + 1, wl * delta_z, acc.err);
-      add_node(g, acc.jz, i_n + 1, k_n + 1, wu * delta_z, acc.err);
+      acc.template add<kCZ>(i_n, k_n + 1, wl * delta_z);
+      acc.template add<kCZ>(i_n + 1, k_n + 1, wu * delta_z);
       delta_z = longitude_new - (k_n + 1) * dz;
-      add_node(g, acc.jz, i_n, k_n, wl * delta_z, acc.err);
-      add_node(g, acc.jz, i_n + 1, k_n, wu * delta_z, acc.err);
+      acc.template add<kCZ>(i_n, k_n, wl * delta_z);
+      acc.template add<kCZ>(i_n + 1, k_n, wu * delta_z);
     }
   } else if (still_z) {
     // strictly radial motion (:890-986)
@@ -367,48 +360,57 @@ __device__ void strict_motion(const GridDesc &g, const Accum &acc, double q, dou
       double ssd = density_strict(q, r0, dr, dz, delta_t);
       double wj = ssd * (delta_r - r0 * r0 / (radius_old + delta_r) + r0 * r0 / radius_old +
                          dr * dr / (4. * (radius_old + delta_r)) - dr * dr / (4. * radius_old));
-      add_node(g, acc.jr, i_n, k_n, wj * left_delta_z, acc.err);
-      add_node(g, acc.jr, i_n, k_n + 1, wj * right_delta_z, acc.err);
+      acc.template add<kCR>(i_n, k_n, wj * left_delta_z);
+      acc.template add<kCR>(i_n, k_n + 1, wj * right_delta_z);
     } else if (res_i == 1) {
       double delta_r = (i_n)*dr - radius_old;
       double r0 = (i_n - 0.5) * dr;
       double ssd = density_strict(q, r0, dr, dz, delta_t);
       double wj = ssd * (delta_r - r0 * r0 / (radius_old + delta_r) + r0 * r0 / radius_old +
                          dr * dr / (4. * (radius_old + delta_r)) - dr * dr / (4. * radius_old));
-      add_node(g, acc.jr, i_n - 1, k_n, wj * left_delta_z, acc.err);
-      add_node(g, acc.jr, i_n - 1, k_n + 1, wj * right_delta_z, acc.err);
+      acc.template add<kCR>(i_n - 1, k_n, wj * left_delta_z);
+      acc.template add<kCR>(i_n - 1, k_n + 1, wj * right_delta_z);
       delta_r = radius_new - i_n * dr;
       r0 = (i_n + 0.5) * dr;
       ssd = density_strict(q, r0, dr, dz, delta_t);
       // `r0 * r0 / i_n * dr` is ((r0*r0)/i_n)*dr in the reference (:941)
       wj = ssd * (delta_r - r0 * r0 / (i_n * dr + delta_r) + r0 * r0 / i_n * dr +
                   dr * dr / (4. * (i_n * dr + delta_r)) - dr * dr / (4. * i_n * dr));
-      add_node(g, acc.jr, i_n, k_n, wj * left_delta_z, acc.err);
-      add_node(g, acc.jr, i_n, k_n + 1, wj * right_delta_z, acc.err);
+      acc.template add<kCR>(i_n, k_n, wj * left_delta_z);
+      acc.template add<kCR>(i_n, k_n + 1, wj * right_delta_z);
     } else if (res_i == -1) {
       double delta_r = (i_n + 1) * dr - radius_old;
       double r0 = (i_n + 1.5) * dr;
       double ssd = density_strict(q, r0, dr, dz, delta_t);
       double wj = ssd * (delta_r - r0 * r0 / (radius_old + delta_r) + r0 * r0 / radius_old +
                          dr * dr / (4. * (radius_old + delta_r)) - dr * dr / (4. * radius_old));
-      add_node(g, acc.jr, i_n + 1, k_n, wj * left_delta_z, acc.err);
-      add_node(g, acc.jr, i_n + 1, k_n + 1, wj * right_delta_z, acc.err);
+      acc.template add<kCR>(i_n + 1, k_n, wj * left_delta_z);
+      acc.template add<kCR>(i_n + 1, k_n + 1, wj * right_delta_z);
       delta_r = radius_new - (i_n + 1) * dr;
       r0 = (i_n + 0.5) * dr;
       ssd = density_strict(q, r0, dr, dz, delta_t);
       // same precedence as above (:976)
       wj = ssd * (delta_r - r0 * r0 / ((i_n + 1) * dr + delta_r) + r0 * r0 / (i_n + 1) * dr +
                   dr * dr / (4. * ((i_n + 1) * dr + delta_r)) - dr * dr / (4. * (i_n + 1) * dr));
-      add_node(g, acc.jr, i_n, k_n, wj * left_delta_z, acc.err);
-      add_node(g, acc.jr, i_n, k_n + 1, wj * right_delta_z, acc.err);
+      acc.template add<kCR>(i_n, k_n, wj * left_delta_z);
+      acc.template add<kCR>(i_n, k_n + 1, wj * right_delta_z);
     }
   }
 }
 
+// one straight piece of a path: segment(rn, zn, ro, zo) inside cell (i, k)
+struct Seg {
+  double rn, zn, ro, zo;
+  int i, k;
+};
+
 // current_distribution for one particle, src/particles.cpp:480-685: pick the
 // cells of both ends, split the straight path at the cell faces it crosses
-// and deposit every piece inside its own cell
-__device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double pr, double pz, double por,
+// and deposit every piece inside its own cell.  The pieces are collected
+// first (with the reference's boundary-point arithmetic) and then deposited
+// by one loop, so that a single copy of segment() is generated.
+template <class S>
+__device__ void deposit_j(const GridDesc &g, S &acc, double q, double pr, double pz, double por,
                           double poz) {
   const double dr = g.dr, dz = g.dz;
   int i_n = cell_number(pr, dr);
@@ -428,37 +430,41 @@ __device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double 
     strict_motion(g, acc, q, pr, pz, por, poz);
     return;
   }
+  Seg s0{pr, pz, por, poz, i_n, k_n}, s1 = s0, s2 = s0;
+  int ns = 0;
   if (res_cell == 0) {
-    segment(g, acc, q, pr, pz, por, poz, i_n, k_n);
+    ns = 1;
   } else if (res_cell == 1) {
     if ((i_n != i_o) && (k_n == k_o)) {
       if (por > (i_n + 1) * dr) {
         double a = (por - pr) / (poz - pz);
         double r_b = (i_n + 1) * dr;
         double z_b = pz + (r_b - pr) / a;
-        segment(g, acc, q, r_b, z_b, por, poz, i_n + 1, k_n);
-        segment(g, acc, q, pr, pz, r_b, z_b, i_n, k_n);
+        s0 = Seg{r_b, z_b, por, poz, i_n + 1, k_n};
+        s1 = Seg{pr, pz, r_b, z_b, i_n, k_n};
       } else {
         double a = (pr - por) / (pz - poz);
         double r_b = (i_n)*dr;
         double z_b = poz + (r_b - por) / a;
-        segment(g, acc, q, r_b, z_b, por, poz, i_n - 1, k_n);
-        segment(g, acc, q, pr, pz, r_b, z_b, i_n, k_n);
+        s0 = Seg{r_b, z_b, por, poz, i_n - 1, k_n};
+        s1 = Seg{pr, pz, r_b, z_b, i_n, k_n};
       }
+      ns = 2;
     } else if ((i_n == i_o) && (k_n != k_o)) {
       if (poz < k_n * dz) {
         double z_b = k_n * dz;
         double a = (pr - por) / (pz - poz);
         double r_b = por + a * (z_b - poz);
-        segment(g, acc, q, r_b, z_b, por, poz, i_n, k_n - 1);
-        segment(g, acc, q, pr, pz, r_b, z_b, i_n, k_n);
+        s0 = Seg{r_b, z_b, por, poz, i_n, k_n - 1};
+        s1 = Seg{pr, pz, r_b, z_b, i_n, k_n};
       } else {
         double z_b = (k_n + 1) * dz;
         double a = (por - pr) / (poz - pz);
         double r_b = pr + a * (z_b - pz);
-        segment(g, acc, q, r_b, z_b, por, poz, i_n, k_n + 1);
-        segment(g, acc, q, pr, pz, r_b, z_b, i_n, k_n);
+        s0 = Seg{r_b, z_b, por, poz, i_n, k_n + 1};
+        s1 = Seg{pr, pz, r_b, z_b, i_n, k_n};
       }
+      ns = 2;
     }
   } else if (res_cell == 2) {
     if (i_o < i_n) {
@@ -469,13 +475,15 @@ __device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double 
         double z2 = k_n * dz;
         double r2 = por + (z2 - poz) * a;
         if (z1 < k_n * dz) {
-          segment(g, acc, q, r1, z1, por, poz, i_n - 1, k_n - 1);
-          segment(g, acc, q, r2, z2, r1, z1, i_n, k_n - 1);
-          segment(g, acc, q, pr, pz, r2, z2, i_n, k_n);
+          s0 = Seg{r1, z1, por, poz, i_n - 1, k_n - 1};
+          s1 = Seg{r2, z2, r1, z1, i_n, k_n - 1};
+          s2 = Seg{pr, pz, r2, z2, i_n, k_n};
+          ns = 3;
         } else if (z1 > k_n * dz) {
-          segment(g, acc, q, r2, z2, por, poz, i_n - 1, k_n - 1);
-          segment(g, acc, q, r1, z1, r2, z2, i_n - 1, k_n);
-          segment(g, acc, q, pr, pz, r1, z1, i_n, k_n);
+          s0 = Seg{r2, z2, por, poz, i_n - 1, k_n - 1};
+          s1 = Seg{r1, z1, r2, z2, i_n - 1, k_n};
+          s2 = Seg{pr, pz, r1, z1, i_n, k_n};
+          ns = 3;
         }
       } else {
         double a = (pr - por) / (pz - poz);
@@ -484,13 +492,15 @@ __device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double 
         double z2 = (k_n + 1) * dz;
         double r2 = por + -(poz - z2) * a;
         if (z1 > (k_n + 1) * dz) {
-          segment(g, acc, q, r1, z1, por, poz, i_n - 1, k_n + 1);
-          segment(g, acc, q, r2, z2, r1, z1, i_n, k_n + 1);
-          segment(g, acc, q, pr, pz, r2, z2, i_n, k_n);
+          s0 = Seg{r1, z1, por, poz, i_n - 1, k_n + 1};
+          s1 = Seg{r2, z2, r1, z1, i_n, k_n + 1};
+          s2 = Seg{pr, pz, r2, z2, i_n, k_n};
+          ns = 3;
         } else if (z1 < (k_n + 1) * dz) {
-          segment(g, acc, q, r2, z2, por, poz, i_n - 1, k_n + 1);
-          segment(g, acc, q, r1, z1, r2, z2, i_n - 1, k_n);
-          segment(g, acc, q, pr, pz, r1, z1, i_n, k_n);
+          s0 = Seg{r2, z2, por, poz, i_n - 1, k_n + 1};
+          s1 = Seg{r1, z1, r2, z2, i_n - 1, k_n};
+          s2 = Seg{pr, pz, r1, z1, i_n, k_n};
+          ns = 3;
         }
       }
     } else if (i_o > i_n) {
@@ -501,13 +511,15 @@ __device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double 
         double z2 = k_n * dz;
         double r2 = por - -(z2 - poz) * a;
         if (z1 < (k_n)*dz) {
-          segment(g, acc, q, r1, z1, por, poz, i_n + 1, k_n - 1);
-          segment(g, acc, q, r2, z2, r1, z1, i_n, k_n - 1);
-          segment(g, acc, q, pr, pz, r2, z2, i_n, k_n);
+          s0 = Seg{r1, z1, por, poz, i_n + 1, k_n - 1};
+          s1 = Seg{r2, z2, r1, z1, i_n, k_n - 1};
+          s2 = Seg{pr, pz, r2, z2, i_n, k_n};
+          ns = 3;
         } else if (z1 > (k_n)*dz) {
-          segment(g, acc, q, r2, z2, por, poz, i_n + 1, k_n - 1);
-          segment(g, acc, q, r1, z1, r2, z2, i_n + 1, k_n);
-          segment(g, acc, q, pr, pz, r1, z1, i_n, k_n);
+          s0 = Seg{r2, z2, por, poz, i_n + 1, k_n - 1};
+          s1 = Seg{r1, z1, r2, z2, i_n + 1, k_n};
+          s2 = Seg{pr, pz, r1, z1, i_n, k_n};
+          ns = 3;
         }
       } else if (k_o > k_n) {
         double a = (por - pr) / (poz - pz);
@@ -516,16 +528,23 @@ __device__ void deposit_j(const GridDesc &g, const Accum &acc, double q, double 
         double z2 = (k_n + 1) * dz;
         double r2 = pr + (z2 - pz) * a;
         if (z1 > (k_n + 1) * dz) {
-          segment(g, acc, q, r1, z1, por, poz, i_n + 1, k_n + 1);
-          segment(g, acc, q, r2, z2, r1, z1, i_n, k_n + 1);
-          segment(g, acc, q, pr, pz, r2, z2, i_n, k_n);
+          s0 = Seg{r1, z1, por, poz, i_n + 1, k_n + 1};
+          s1 = Seg{r2, z2, r1, z1, i_n, k_n + 1};
+          s2 = Seg{pr, pz, r2, z2, i_n, k_n};
+          ns = 3;
         } else if (z1 < (k_n + 1) * dz) {
-          segment(g, acc, q, r2, z2, por, poz, i_n + 1, k_n + 1);
-          segment(g, acc, q, r1, z1, r2, z2, i_n + 1, k_n);
-          segment(g, acc, q, pr, pz, r1, z1, i_n, k_n);
+          s0 = Seg{r2, z2, por, poz, i_n + 1, k_n + 1};
+          s1 = Seg{r1, z1, r2, z2, i_n + 1, k_n};
+          s2 = Seg{pr, pz, r1, z1, i_n, k_n};
+          ns = 3;
         }
       }
     }
+  }
+#pragma nounroll
+  for (int t = 0; t < ns; t++) {
+    const Seg c = t == 0 ? s0 : (t == 1 ? s1 : s2);
+    segment(g, acc, q, c.rn, c.zn, c.ro, c.zo, c.i, c.k);
   }
 }
 
@@ -590,11 +609,12 @@ __device__ bool push_half(const GridDesc &g, const Fields &f, int pusher, int *e
 }
 
 // full_current_distribution part for one particle (src/particles.cpp:1367-1373)
-__device__ bool deposit_move(const GridDesc &g, const Accum &acc, bool is_bunch, bool want_rho, double q,
+template <class S>
+__device__ bool deposit_move(const GridDesc &g, S &sk, int *err, bool is_bunch, bool want_rho, double q,
                              double &r, double &z, double v[3], bool &alive, double r_old, double z_old) {
-  deposit_jphi(g, acc, q, r, z, v[1]);
+  deposit_jphi(g, sk, q, r, z, v[1]);
   if (!finite2(r, z)) {
-    atomicOr(acc.err, kErrNonfinite);
+    atomicOr(err, kErrNonfinite);
     return false;
   }
   const double half_dt = g.dt / 2.;
@@ -606,31 +626,34 @@ __device__ bool deposit_move(const GridDesc &g, const Accum &acc, bool is_bunch,
   double st = y / rr;
   r = rr;
   if (alive) {
-    deposit_j(g, acc, q, r, z, r_old, z_old);
+    deposit_j(g, sk, q, r, z, r_old, z_old);
     double ct = sqrt(1 - st * st);
     double u2 = ct * v[0] - st * v[1];
     double w2 = st * v[0] + ct * v[1];
     v[0] = u2;
     v[1] = w2;
-    if (want_rho) deposit_rho(g, acc.rho, acc.err, q, r, z);
+    if (want_rho) deposit_rho(g, sk, q, r, z);
   }
   return true;
 }
 
-__global__ void __launch_bounds__(256) k_fused(GridDesc g, Fields f, Accum acc, SpeciesArgs s, int pusher,
-                                               int want_rho) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  if (!s.alive[i]) return;
+// the whole step of one particle in slot i (push, both half steps, deposits)
+template <class S>
+__device__ __forceinline__ void step_particle(const GridDesc &g, const Fields &f, S &sk, int *err,
+                                              const SpeciesArgs &s, int pusher, bool want_rho, long long i) {
+  if (s.alive[i] != 1) return;
   double r = s.r[i], z = s.z[i];
   double v[3] = {s.vr[i], s.vphi[i], s.vz[i]};
   const double w = s.w[i];
   const double q = s.charge * w, m = s.mass * w;
   const double r_old = r, z_old = z;
+  // contributions of this particle are centred on its starting cell
+  int ci = cell_number(r, g.dr), ck = cell_number(z, g.dz);
+  sk.begin_cell(ci < 0 ? 0 : ci, ck < 0 ? 0 : ck);
   double sin_t;
-  if (!push_half(g, f, pusher, acc.err, q, m, r, z, v, sin_t)) return;
+  if (!push_half(g, f, pusher, err, q, m, r, z, v, sin_t)) return;
   bool alive = true;
-  if (!deposit_move(g, acc, s.is_bunch, want_rho, q, r, z, v, alive, r_old, z_old)) return;
+  if (!deposit_move(g, sk, err, s.is_bunch, want_rho, q, r, z, v, alive, r_old, z_old)) return;
   s.r[i] = r;
   s.z[i] = z;
   s.vr[i] = v[0];
@@ -639,10 +662,46 @@ __global__ void __launch_bounds__(256) k_fused(GridDesc g, Fields f, Accum acc, 
   if (!alive) s.alive[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_step_v(GridDesc g, Fields f, int *err, SpeciesArgs s, int pusher) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  if (!s.alive[i]) return;
+// chunked plasma species: 4 wavefronts per block, each walking one chunk of
+// kChunk slots kPPT particles per lane, deposits through the tile sink
+__global__ void __launch_bounds__(256) k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
+                                                     long long n_cap, int pusher) {
+  __shared__ double box[kBoxSize];
+  const long long base = (long long)blockIdx.x * kBlockParticles;
+  // anchor: the block's first logical particle (smallest cell of a sorted block)
+  int ai = 0, ak = 0;
+  if (base < n_cap && s.alive[base] == 1) {
+    ai = max(0, cell_number(s.r[base], g.dr));
+    ak = max(0, cell_number(s.z[base], g.dz));
+  }
+  TileSink sk;
+  box_init(box, ai, ak, sk, gs);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long cbase = base + (long long)wave * kChunk + lane;
+  for (int j = 0; j < kPPT; j++) {
+    long long p = cbase + (long long)j * 64;
+    if (p < n_cap) step_particle(g, f, sk, gs.err, s, pusher, false, p);
+  }
+  sk.flush();
+  __syncthreads();
+  box_flush(box, sk);
+}
+
+// plain layout (bunches): one particle per thread, deposits straight to HBM
+__global__ void __launch_bounds__(256) k_fused_plain(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
+                                                     long long n_cap, int pusher, int want_rho) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cap) return;
+  GlobalSink sk = gs;
+  step_particle(g, f, sk, gs.err, s, pusher, want_rho != 0, i);
+}
+
+__global__ void __launch_bounds__(256) k_step_v(GridDesc g, Fields f, int *err, SpeciesArgs s, long long n_cap,
+                                                int pusher) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cap) return;
+  if (s.alive[i] != 1) return;
   double r = s.r[i], z = s.z[i];
   double v[3] = {s.vr[i], s.vphi[i], s.vz[i]};
   const double w = s.w[i];
@@ -658,15 +717,16 @@ __global__ void __launch_bounds__(256) k_step_v(GridDesc g, Fields f, int *err, 
   s.sin_t[i] = sin_t;
 }
 
-__global__ void __launch_bounds__(256) k_full_current(GridDesc g, Accum acc, SpeciesArgs s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  if (!s.alive[i]) return;
+__global__ void __launch_bounds__(256) k_full_current(GridDesc g, GlobalSink gs, SpeciesArgs s, long long n_cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cap) return;
+  if (s.alive[i] != 1) return;
   double r = s.r[i], z = s.z[i];
   double v[3] = {s.vr[i], s.vphi[i], s.vz[i]};
   const double q = s.charge * s.w[i];
   bool alive = true;
-  if (!deposit_move(g, acc, s.is_bunch, false, q, r, z, v, alive, s.r_old[i], s.z_old[i])) return;
+  GlobalSink sk = gs;
+  if (!deposit_move(g, sk, gs.err, s.is_bunch, false, q, r, z, v, alive, s.r_old[i], s.z_old[i])) return;
   s.r[i] = r;
   s.z[i] = z;
   s.vr[i] = v[0];
@@ -675,16 +735,17 @@ __global__ void __launch_bounds__(256) k_full_current(GridDesc g, Accum acc, Spe
   if (!alive) s.alive[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_charge(GridDesc g, double *rho, int *err, SpeciesArgs s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  if (!s.alive[i]) return;
+__global__ void __launch_bounds__(256) k_charge(GridDesc g, GlobalSink gs, SpeciesArgs s, long long n_cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cap) return;
+  if (s.alive[i] != 1) return;
   double r = s.r[i], z = s.z[i];
   if (!finite2(r, z)) {
-    atomicOr(err, kErrNonfinite);
+    atomicOr(gs.err, kErrNonfinite);
     return;
   }
-  deposit_rho(g, rho, err, s.charge * s.w[i], r, z);
+  GlobalSink sk = gs;
+  deposit_rho(g, sk, s.charge * s.w[i], r, z);
 }
 
 __global__ void k_gather(GridDesc g, Fields f, int which, const double *r, const double *z, int n, double *out,
@@ -717,6 +778,18 @@ Fields fields_of(Ctx &c) {
   return Fields{c.a[kER], c.a[kEPhi], c.a[kEZ], c.a[kHHR], c.a[kHHPhi], c.a[kHHZ]};
 }
 
+GlobalSink sink_of(Ctx &c) {
+  GlobalSink gs;
+  gs.nr = c.g.nr;
+  gs.nz = c.g.nz;
+  gs.jr = c.a[kJR];
+  gs.jphi = c.a[kJPhi];
+  gs.jz = c.a[kJZ];
+  gs.rho = c.a[kRhoBunch];
+  gs.err = c.err;
+  return gs;
+}
+
 SpeciesArgs args_of(Species &sp) {
   SpeciesArgs s;
   s.r = sp.f64.r;
@@ -736,33 +809,38 @@ SpeciesArgs args_of(Species &sp) {
   return s;
 }
 
-unsigned blocks_for(int n) { return (unsigned)((n + 255) / 256); }
+unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
 void launch_fused_f64(Ctx &c, Species &sp) {
-  if (sp.n <= 0) return;
-  Accum acc{c.a[kJR], c.a[kJPhi], c.a[kJZ], c.a[kRhoBunch], c.err};
-  hipLaunchKernelGGL(k_fused, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, fields_of(c), acc,
-                     args_of(sp), c.pusher, sp.kind == 1 ? 1 : 0);
+  if (sp.n_cap <= 0) return;
+  if (sp.chunked) {
+    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
+    hipLaunchKernelGGL(k_fused_tiled, dim3(blocks), dim3(256), 0, c.stream, c.g, fields_of(c), sink_of(c),
+                       args_of(sp), sp.n_cap, c.pusher);
+  } else {
+    hipLaunchKernelGGL(k_fused_plain, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, fields_of(c),
+                       sink_of(c), args_of(sp), sp.n_cap, c.pusher, sp.kind == 1 ? 1 : 0);
+  }
 }
 
 void launch_step_v_f64(Ctx &c, Species &sp) {
-  if (sp.n <= 0) return;
-  hipLaunchKernelGGL(k_step_v, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, fields_of(c), c.err,
-                     args_of(sp), c.pusher);
+  if (sp.n_cap <= 0) return;
+  hipLaunchKernelGGL(k_step_v, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, fields_of(c), c.err,
+                     args_of(sp), sp.n_cap, c.pusher);
 }
 
 void launch_full_current_f64(Ctx &c, Species &sp) {
-  if (sp.n <= 0) return;
-  Accum acc{c.a[kJR], c.a[kJPhi], c.a[kJZ], c.a[kRhoBunch], c.err};
-  hipLaunchKernelGGL(k_full_current, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, acc, args_of(sp));
+  if (sp.n_cap <= 0) return;
+  hipLaunchKernelGGL(k_full_current, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, sink_of(c),
+                     args_of(sp), sp.n_cap);
 }
 
 void launch_charge_weighting_f64(Ctx &c, Species &sp) {
-  if (sp.n <= 0) return;
-  hipLaunchKernelGGL(k_charge, dim3(blocks_for(sp.n)), dim3(256), 0, c.stream, c.g, c.a[kRhoBunch], c.err,
-                     args_of(sp));
+  if (sp.n_cap <= 0) return;
+  hipLaunchKernelGGL(k_charge, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, sink_of(c), args_of(sp),
+                     sp.n_cap);
 }
 
 void launch_gather_f64(Ctx &c, int which, const double *r, const double *z, int n, double *out) {

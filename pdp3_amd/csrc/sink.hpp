@@ -1,0 +1,161 @@
+// Where the particle kernels put their grid contributions (J_r, J_phi, J_z
+// and the bunch rho, all FP64).
+//
+// GlobalSink: bounds-checked hardware FP64 atomics straight to HBM (used by
+// the split entry points and by bunches, whose particles are not sorted).
+//
+// TileSink: three levels, for cell-sorted particles walked by one lane in
+// runs of the same cell:
+//   1. registers: the contributions of the current cell (ic, kc) to its
+//      J_phi / rho corners (ic..ic+1, kc..kc+1), J_z nodes (ic..ic+1, kc) and
+//      J_r nodes (ic, kc..kc+1) are summed in FP64 registers and flushed only
+//      when the lane moves to a particle in another cell;
+//   2. an LDS box of kBoxR x kBoxK nodes per component anchored at the
+//      block's first particle, filled with LDS FP64 atomics;
+//   3. nodes outside the box go to GlobalSink.
+// After the block the box is added to HBM once (one atomic per touched node).
+#pragma once
+
+#include "context.hpp"
+
+namespace pdp3 {
+
+enum SinkComp { kCR = 0, kCPhi = 1, kCZ = 2, kCRho = 3 };
+
+struct GlobalSink {
+  int nr, nz;
+  double *jr, *jphi, *jz, *rho;
+  int *err;
+  __device__ __forceinline__ double *base(int comp) const {
+    return comp == kCR ? jr : comp == kCPhi ? jphi : comp == kCZ ? jz : rho;
+  }
+  __device__ __forceinline__ void put(int comp, int i, int k, double v) const {
+    if ((unsigned)i < (unsigned)nr && (unsigned)k < (unsigned)nz)
+      unsafeAtomicAdd(base(comp) + (size_t)i * (size_t)nz + (size_t)k, v);
+    else
+      atomicOr(err, kErrDomain);
+  }
+  template <int C>
+  __device__ __forceinline__ void add(int i, int k, double v) {
+    put(C, i, k, v);
+  }
+  __device__ __forceinline__ void begin_cell(int, int) {}
+  __device__ __forceinline__ void flush() {}
+};
+
+constexpr int kBoxR = 4;    // rows i0-1 .. i0+2 of the anchor cell
+constexpr int kBoxK = 128;  // columns k0-2 .. k0+125
+constexpr int kBoxComps = 4;
+constexpr int kBoxSize = kBoxComps * kBoxR * kBoxK;
+
+template <bool RHO>
+struct TileSinkT {
+  GlobalSink g;
+  double *box;  // LDS, kBoxComps x kBoxR x kBoxK
+  int bi, bk;   // node coordinates of box[.][0][0]
+  int ic, kc;   // cell of the register accumulator
+  // scalar accumulators (no arrays: a runtime index would push them to scratch)
+  double p00, p10, p01, p11;  // J_phi corners (ic+di, kc+dk)
+  double r00, r10, r01, r11;  // rho corners (bunches only)
+  double z0, z1;              // J_z at (ic, kc), (ic+1, kc)
+  double x0, x1;              // J_r at (ic, kc), (ic, kc+1)
+
+  __device__ __forceinline__ void box_put(int comp, int i, int k, double v) {
+    int li = i - bi, lk = k - bk;
+    if ((unsigned)li < (unsigned)kBoxR && (unsigned)lk < (unsigned)kBoxK)
+      atomicAdd(box + (comp * kBoxR + li) * kBoxK + lk, v);
+    else
+      g.put(comp, i, k, v);
+  }
+  template <int C>
+  __device__ __forceinline__ void add(int i, int k, double v) {
+    const int di = i - ic, dk = k - kc;
+    if (C == kCPhi || (C == kCRho && RHO)) {
+      if ((unsigned)di < 2u && (unsigned)dk < 2u) {
+        const double a00 = (di == 0 && dk == 0) ? v : 0.0, a10 = (di == 1 && dk == 0) ? v : 0.0;
+        const double a01 = (di == 0 && dk == 1) ? v : 0.0, a11 = (di == 1 && dk == 1) ? v : 0.0;
+        if (C == kCPhi) {
+          p00 += a00; p10 += a10; p01 += a01; p11 += a11;
+        } else {
+          r00 += a00; r10 += a10; r01 += a01; r11 += a11;
+        }
+        return;
+      }
+    } else if (C == kCZ) {
+      if (dk == 0 && (unsigned)di < 2u) {
+        z0 += di == 0 ? v : 0.0;
+        z1 += di == 1 ? v : 0.0;
+        return;
+      }
+    } else if (C == kCR) {
+      if (di == 0 && (unsigned)dk < 2u) {
+        x0 += dk == 0 ? v : 0.0;
+        x1 += dk == 1 ? v : 0.0;
+        return;
+      }
+    }
+    box_put(C, i, k, v);
+  }
+  __device__ __forceinline__ void put_nz(int comp, int i, int k, double &v) {
+    if (v != 0.0) box_put(comp, i, k, v);
+    v = 0.0;
+  }
+  __device__ __forceinline__ void flush() {
+    if (ic < 0) return;
+    put_nz(kCPhi, ic, kc, p00);
+    put_nz(kCPhi, ic + 1, kc, p10);
+    put_nz(kCPhi, ic, kc + 1, p01);
+    put_nz(kCPhi, ic + 1, kc + 1, p11);
+    if (RHO) {
+      put_nz(kCRho, ic, kc, r00);
+      put_nz(kCRho, ic + 1, kc, r10);
+      put_nz(kCRho, ic, kc + 1, r01);
+      put_nz(kCRho, ic + 1, kc + 1, r11);
+    }
+    put_nz(kCZ, ic, kc, z0);
+    put_nz(kCZ, ic + 1, kc, z1);
+    put_nz(kCR, ic, kc, x0);
+    put_nz(kCR, ic, kc + 1, x1);
+  }
+  // the next particle's contributions are centred on cell (i, k)
+  __device__ __forceinline__ void begin_cell(int i, int k) {
+    if (i != ic || k != kc) {
+      flush();
+      ic = i;
+      kc = k;
+    }
+  }
+};
+using TileSink = TileSinkT<false>;
+
+// block prologue: anchor the box at the cell of the block's first logical
+// particle (the smallest key of a sorted block) and clear it
+template <bool RHO>
+__device__ __forceinline__ void box_init(double *box, int anchor_i, int anchor_k, TileSinkT<RHO> &s,
+                                         const GlobalSink &g) {
+  for (int t = threadIdx.x; t < kBoxSize; t += blockDim.x) box[t] = 0.0;
+  s.g = g;
+  s.box = box;
+  s.bi = anchor_i - 1;
+  s.bk = anchor_k - 2;
+  s.ic = -1;
+  s.kc = -1;
+  s.p00 = s.p10 = s.p01 = s.p11 = 0.0;
+  s.r00 = s.r10 = s.r01 = s.r11 = 0.0;
+  s.z0 = s.z1 = s.x0 = s.x1 = 0.0;
+}
+
+// block epilogue: add the box to the grid (after __syncthreads)
+template <bool RHO>
+__device__ __forceinline__ void box_flush(const double *box, const TileSinkT<RHO> &s) {
+  for (int t = threadIdx.x; t < kBoxSize; t += blockDim.x) {
+    double v = box[t];
+    if (v == 0.0) continue;
+    int comp = t / (kBoxR * kBoxK);
+    int rem = t - comp * kBoxR * kBoxK;
+    int li = rem / kBoxK, lk = rem - li * kBoxK;
+    s.g.put(comp, s.bi + li, s.bk + lk, v);
+  }
+}
+
+}  // namespace pdp3

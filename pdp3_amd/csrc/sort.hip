@@ -1,9 +1,10 @@
 // Stable counting sort of a species' particles by cell (no reference
 // counterpart: it keeps gathers and deposits of neighbouring particles on the
 // same cache lines).  Key = i * (nz-1) + k of the reference's cell index
-// (include/geometry.h:44, clamped at 0); dead particles get the largest key
-// and end up behind the living ones.  The order among equal keys is the
-// input order, so the result equals a stable sort by key.
+// (include/geometry.h:44, clamped at 0); dead particles get the next key and
+// end up behind the living ones, chunk padding slots after those.  The order
+// among equal keys is the input order, so the result equals a stable sort by
+// key; it is written back in the chunk-transposed layout.
 //
 // LSD radix sort with 8-bit digits (one counting sort per digit):
 //   hist:    per block of 16384 elements, a 256-bin digit histogram
@@ -126,36 +127,40 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t *__restrict
   }
 }
 
+// keys in logical order (so that ties keep their logical order); vals = slot
 __global__ void k_keys_f64(GridDesc g, const double *r, const double *z, const uint8_t *alive, long long n,
                            uint32_t *keys, uint32_t *vals, uint32_t dead_key) {
-  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  uint32_t key = dead_key;
-  if (alive[j]) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  long long j = phys_of(s);
+  uint32_t key = alive[j] == kPadSlot ? dead_key + 1 : dead_key;
+  if (alive[j] == 1) {
     int i = (int)ceil(r[j] / g.dr) - 1, k = (int)ceil(z[j] / g.dz) - 1;
     i = max(0, min(i, g.nr - 2));
     k = max(0, min(k, g.nz - 2));
     key = (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
   }
-  keys[j] = key;
-  vals[j] = (uint32_t)j;
+  keys[s] = key;
+  vals[s] = (uint32_t)j;
 }
 
 __global__ void k_keys_f32(GridDesc g, const uint32_t *cell, long long n, uint32_t *keys, uint32_t *vals,
                            uint32_t dead_key) {
-  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  long long j = phys_of(s);
   uint32_t c = cell[j];
-  uint32_t key = dead_key;
+  uint32_t key = c == kPadCell ? dead_key + 1 : dead_key;
   if (!(c & 0x80000000u)) key = ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
-  keys[j] = key;
-  vals[j] = (uint32_t)j;
+  keys[s] = key;
+  vals[s] = (uint32_t)j;
 }
 
+// slot p of the chunked layout holds sorted rank logical_of(p)
 template <typename T>
 __global__ void k_gather(const T *__restrict__ in, T *__restrict__ out, const uint32_t *__restrict__ perm, long long n) {
-  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) out[j] = in[perm[j]];
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = in[perm[logical_of(p)]];
 }
 
 void exclusive_scan(uint32_t *data, long long n, uint32_t *tmp, hipStream_t s) {
@@ -185,12 +190,12 @@ bool gather_swap(T *&a, T *&alt, const uint32_t *perm, long long n, hipStream_t 
 }  // namespace
 
 int sort_species(Ctx &c, Species &sp) {
-  const long long n = sp.n;
-  if (n <= 1) return 0;
+  const long long n = sp.n_cap;
+  if (n <= 1 || !sp.chunked) return 0;
   const uint32_t ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
   const uint32_t dead_key = ncell;
   int bits = 1;
-  while ((1ull << bits) <= (unsigned long long)dead_key) bits++;
+  while ((1ull << bits) <= (unsigned long long)dead_key + 1) bits++;
   const int passes = (bits + 7) / 8;
   const int nblocks = (int)((n + kPerBlock - 1) / kPerBlock);
   const long long hist_n = (long long)kBins * nblocks;
