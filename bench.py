@@ -93,6 +93,25 @@ def cpu_baseline(wl, capi, w, threads):
                        f"({s.plasma[0].ppc} ppc), {nsteps} full steps, {dt:.1f} s")
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of nbytes through the HIP runtime, for
+    scaling the PMC byte counters (FETCH_SIZE / WRITE_SIZE) of the particle
+    kernel: the counters of this copy are compared with 2 x nbytes moved."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +121,12 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

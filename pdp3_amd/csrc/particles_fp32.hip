@@ -24,6 +24,10 @@ namespace {
 
 constexpr uint32_t kDead = 0x80000000u;
 
+// a / b through the hardware reciprocal (about 1 ulp; inputs are never
+// denormal here)
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 __device__ __forceinline__ int cell_i(uint32_t c) { return (int)((c >> 16) & 0x7FFF); }
 __device__ __forceinline__ int cell_k(uint32_t c) { return (int)(c & 0xFFFF); }
 __device__ __forceinline__ uint32_t pack(int i, int k) { return ((uint32_t)i << 16) | (uint32_t)k; }
@@ -71,8 +75,8 @@ __device__ __forceinline__ RadW rad_lo(int i, float a) {
   float fi = (float)ir;
   RadW w;
   w.ir = ir;
-  w.A = (1.f - al) * (2.f * fi + 1.f + al) / (2.f * (fi + 1.f));
-  w.B = al * (2.f * fi + 2.f + al) / (2.f * (fi + 3.f));
+  w.A = fdiv((1.f - al) * (2.f * fi + 1.f + al), 2.f * (fi + 1.f));
+  w.B = fdiv(al * (2.f * fi + 2.f + al), 2.f * (fi + 3.f));
   return w;
 }
 __device__ __forceinline__ RadW rad_own(int i, float a) {
@@ -84,8 +88,8 @@ __device__ __forceinline__ RadW rad_own(int i, float a) {
     w.B = (float)(ad * (1.0 + ad) / 4.0);
   } else {
     float fi = (float)i;
-    w.A = (1.f - a) * (2.f * fi + a) / (2.f * fi);
-    w.B = a * (2.f * fi + 1.f + a) / (2.f * (fi + 2.f));
+    w.A = fdiv((1.f - a) * (2.f * fi + a), 2.f * fi);
+    w.B = fdiv(a * (2.f * fi + 1.f + a), 2.f * (fi + 2.f));
   }
   return w;
 }
@@ -131,7 +135,7 @@ __device__ __forceinline__ void boris_u(const F32Args &p, const float e[3], cons
   for (int j = 0; j < 3; j++) u[j] += ee[j];
   float g = rsqrtf(1.f + dot3f(u, u) * p.inv_c2);
   for (int j = 0; j < 3; j++) b[j] *= g;
-  float c2 = 2.f / (1.f + dot3f(b, b));
+  float c2 = fdiv(2.f, 1.f + dot3f(b, b));
   float t0 = u[0], t1 = u[1], t2 = u[2];
   u[0] = t0 + c2 * ((t1 - t0 * b[2] + t2 * b[0]) * b[2] - (t2 + t0 * b[1] - t1 * b[0]) * b[1]);
   u[1] = t1 + c2 * (-(t0 + t1 * b[2] - t2 * b[1]) * b[2] + (t2 + t0 * b[1] - t1 * b[0]) * b[0]);
@@ -156,7 +160,7 @@ __device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const 
   float us2 = ub * ub;
   float ig = rsqrtf(0.5f * (s + sqrtf(s * s + 4.f * (B2 + us2))));  // 1/gamma^{n+1}
   for (int j = 0; j < 3; j++) b[j] *= ig;
-  s = 1.f / (1.f + dot3f(b, b));
+  s = fdiv(1.f, 1.f + dot3f(b, b));
   alpha = dot3f(up, b);
   u[0] = s * (up[0] + alpha * b[0] + b[2] * up[1] - b[1] * up[2]);
   u[1] = s * (up[1] + alpha * b[1] + b[0] * up[2] - b[2] * up[0]);
@@ -179,8 +183,8 @@ __device__ __forceinline__ void renorm(int &c, float &o) {
 __device__ __forceinline__ float rotate_back(int i, float &xa, float dy) {
   float X = (float)i + xa;
   float R = sqrtf(X * X + dy * dy);
-  xa += dy * dy / (R + X);
-  return dy / R;  // sin(theta)
+  xa += fdiv(dy * dy, R + X);
+  return fdiv(dy, R);  // sin(theta)
 }
 
 // J_phi of one particle at the half-step position,
@@ -194,8 +198,8 @@ __device__ void deposit_jphi(const F32Args &p, S &sk, float w, int i, float a, i
     c1 = 1.0 + ad;
   } else {
     float fi = (float)i, rho = fi + a;
-    c0 = (1.f - a) * (2.f * fi + a) / (rho * fi);
-    c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
+    c0 = fdiv((1.f - a) * (2.f * fi + a), rho * fi);
+    c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
   }
   float s = p.kp * w * vphi;
   sk.template add<kCPhi>(i, k, (double)(s * (1.f - b)) * c0);
@@ -219,8 +223,8 @@ __device__ void deposit_rho(const F32Args &p, S &sk, float w, int i, float a, in
     }
   } else {
     float fi = (float)i, rho = fi + a;
-    c0 = (1.f - a) * (2.f * fi + a) / (rho * fi);
-    c1 = a * (2.f * fi + 1.f + a) / (rho * (fi + 1.f));
+    c0 = fdiv((1.f - a) * (2.f * fi + a), rho * fi);
+    c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
   }
   float s = p.kp * w;
   sk.template add<kCRho>(i, k, (double)(s * (1.f - b)) * c0);
@@ -233,7 +237,7 @@ __device__ void deposit_rho(const F32Args &p, S &sk, float w, int i, float a, in
 __device__ __forceinline__ float m_of(float x) {
   if (fabsf(x) < 0.1f)
     return x * (-0.5f + x * (1.f / 3.f + x * (-0.25f + x * (0.2f + x * (-1.f / 6.f + x * (1.f / 7.f - x * 0.125f))))));
-  return (log1pf(x) - x) / x;
+  return fdiv(log1pf(x) - x, x);
 }
 
 // one straight piece of the path inside cell (i, k): simple_current_distribution
@@ -259,16 +263,18 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
   } else {
     float fi = (float)i;
     float rs = fi + a, re = rs + drho;
-    float x = drho / rs;
+    float irs = __builtin_amdgcn_rcpf(rs);
+    float x = drho * irs;
     float M = m_of(x);
-    float N = -x / (1.f + x) - M;
+    float N = -fdiv(x, 1.f + x) - M;
     float ii = fi * (fi + 1.f);
-    jz0 = ((1.f - a) * (2.f * fi + a) / rs - 0.5f * drho + ii / rs * M) / fi;
-    jz1 = (a * (2.f * fi + 1.f + a) / rs + 0.5f * drho - ii / rs * M) / (fi + 1.f);
-    float cc = ii / (2.f * rs);
-    float inv = 1.f / (fi + 0.5f);
-    jr0 = (drho * (0.5f * (1.f - b) - 0.25f * dzeta) + cc * (drho * (1.f - b) / re + dzeta * N)) * inv;
-    jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b / re - dzeta * N)) * inv;
+    jz0 = fdiv((1.f - a) * (2.f * fi + a) * irs - 0.5f * drho + ii * irs * M, fi);
+    jz1 = fdiv(a * (2.f * fi + 1.f + a) * irs + 0.5f * drho - ii * irs * M, fi + 1.f);
+    float cc = 0.5f * ii * irs;
+    float inv = __builtin_amdgcn_rcpf(fi + 0.5f);
+    float ire = __builtin_amdgcn_rcpf(re);
+    jr0 = (drho * (0.5f * (1.f - b) - 0.25f * dzeta) + cc * (drho * (1.f - b) * ire + dzeta * N)) * inv;
+    jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b * ire - dzeta * N)) * inv;
   }
   float sz = p.kz * w * dzeta, sr = p.kr * w;
   sk.template add<kCZ>(i, k, (double)(sz * jz0));
@@ -430,7 +436,7 @@ __device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, long lo
 // chunked plasma species: 4 wavefronts per block, each walking one chunk of
 // kChunk slots kPPT particles per lane, deposits through the tile sink
 __global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
-  __shared__ double box[kBoxSize];
+  __shared__ double box[TileSink::kSize];
   const long long base = (long long)blockIdx.x * kBlockParticles;
   int ai = 0, ak = 0;
   if (base < n_cap) {
@@ -445,6 +451,38 @@ __global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs,
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long long cbase = base + (long long)wave * kChunk + lane;
+  for (int j = 0; j < kPPT; j++) {
+    long long n = cbase + (long long)j * 64;
+    if (n < n_cap) step_particle32(p, sk, n);
+  }
+  sk.flush();
+  __syncthreads();
+  box_flush(box, sk);
+}
+
+// bunches (plain layout): the same walk, the box anchored at the smallest
+// living cell of the block
+__global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs, long long n_cap) {
+  __shared__ double box[BunchSink::kSize];
+  const long long base = (long long)blockIdx.x * kBlockParticles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long cbase = base + (long long)wave * kChunk + lane;
+  int ai = 0x7fffffff, ak = 0x7fffffff;
+  for (int j = 0; j < kPPT; j++) {
+    long long n = cbase + (long long)j * 64;
+    if (n < n_cap) {
+      uint32_t c = p.cell[n];
+      if (!(c & kDead)) {
+        ai = min(ai, cell_i(c));
+        ak = min(ak, cell_k(c));
+      }
+    }
+  }
+  block_min2(ai, ak);
+  if (ai == 0x7fffffff) return;  // no living particle in the block
+  BunchSink sk;
+  box_init(box, ai, ak, sk, gs);
+  __syncthreads();
   for (int j = 0; j < kPPT; j++) {
     long long n = cbase + (long long)j * 64;
     if (n < n_cap) step_particle32(p, sk, n);
@@ -621,8 +659,8 @@ void launch_fused_f32(Ctx &c, Species &sp) {
     unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
     hipLaunchKernelGGL(k_fused32_tiled, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);
   } else {
-    hipLaunchKernelGGL(k_fused32_plain, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, args32(c, sp),
-                       sink32(c), sp.n_cap);
+    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
+    hipLaunchKernelGGL(k_fused32_bunch, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);
   }
 }
 

@@ -666,7 +666,7 @@ __device__ __forceinline__ void step_particle(const GridDesc &g, const Fields &f
 // kChunk slots kPPT particles per lane, deposits through the tile sink
 __global__ void __launch_bounds__(256) k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
                                                      long long n_cap, int pusher) {
-  __shared__ double box[kBoxSize];
+  __shared__ double box[TileSink::kSize];
   const long long base = (long long)blockIdx.x * kBlockParticles;
   // anchor: the block's first logical particle (smallest cell of a sorted block)
   int ai = 0, ak = 0;
@@ -688,7 +688,37 @@ __global__ void __launch_bounds__(256) k_fused_tiled(GridDesc g, Fields f, Globa
   box_flush(box, sk);
 }
 
-// plain layout (bunches): one particle per thread, deposits straight to HBM
+// bunches (plain layout): the tiled walk with the box anchored at the
+// smallest living cell of the block
+__global__ void __launch_bounds__(256) k_fused_bunch(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
+                                                     long long n_cap, int pusher) {
+  __shared__ double box[BunchSink::kSize];
+  const long long base = (long long)blockIdx.x * kBlockParticles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long cbase = base + (long long)wave * kChunk + lane;
+  int ai = 0x7fffffff, ak = 0x7fffffff;
+  for (int j = 0; j < kPPT; j++) {
+    long long p = cbase + (long long)j * 64;
+    if (p < n_cap && s.alive[p] == 1) {
+      ai = min(ai, max(0, cell_number(s.r[p], g.dr)));
+      ak = min(ak, max(0, cell_number(s.z[p], g.dz)));
+    }
+  }
+  block_min2(ai, ak);
+  if (ai == 0x7fffffff) return;
+  BunchSink sk;
+  box_init(box, ai, ak, sk, gs);
+  __syncthreads();
+  for (int j = 0; j < kPPT; j++) {
+    long long p = cbase + (long long)j * 64;
+    if (p < n_cap) step_particle(g, f, sk, gs.err, s, pusher, true, p);
+  }
+  sk.flush();
+  __syncthreads();
+  box_flush(box, sk);
+}
+
+// plain layout, one particle per thread, deposits straight to HBM
 __global__ void __launch_bounds__(256) k_fused_plain(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
                                                      long long n_cap, int pusher, int want_rho) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -820,8 +850,9 @@ void launch_fused_f64(Ctx &c, Species &sp) {
     hipLaunchKernelGGL(k_fused_tiled, dim3(blocks), dim3(256), 0, c.stream, c.g, fields_of(c), sink_of(c),
                        args_of(sp), sp.n_cap, c.pusher);
   } else {
-    hipLaunchKernelGGL(k_fused_plain, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, fields_of(c),
-                       sink_of(c), args_of(sp), sp.n_cap, c.pusher, sp.kind == 1 ? 1 : 0);
+    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
+    hipLaunchKernelGGL(k_fused_bunch, dim3(blocks), dim3(256), 0, c.stream, c.g, fields_of(c), sink_of(c),
+                       args_of(sp), sp.n_cap, c.pusher);
   }
 }
 

@@ -43,13 +43,15 @@ struct GlobalSink {
   __device__ __forceinline__ void flush() {}
 };
 
-constexpr int kBoxR = 4;    // rows i0-1 .. i0+2 of the anchor cell
-constexpr int kBoxK = 128;  // columns k0-2 .. k0+125
 constexpr int kBoxComps = 4;
-constexpr int kBoxSize = kBoxComps * kBoxR * kBoxK;
 
-template <bool RHO>
+// BR x BK nodes per component; the plasma box spans 4 rows (i0-1 .. i0+2) by
+// 128 columns (a sorted block covers a run of cells along one row), the bunch
+// box 16 rows by 32 columns (a bunch block is a thin disc of particles
+// injected in the same step)
+template <bool RHO, int BR = 4, int BK = 128>
 struct TileSinkT {
+  static constexpr int kR = BR, kK = BK, kSize = kBoxComps * BR * BK;
   GlobalSink g;
   double *box;  // LDS, kBoxComps x kBoxR x kBoxK
   int bi, bk;   // node coordinates of box[.][0][0]
@@ -62,8 +64,8 @@ struct TileSinkT {
 
   __device__ __forceinline__ void box_put(int comp, int i, int k, double v) {
     int li = i - bi, lk = k - bk;
-    if ((unsigned)li < (unsigned)kBoxR && (unsigned)lk < (unsigned)kBoxK)
-      atomicAdd(box + (comp * kBoxR + li) * kBoxK + lk, v);
+    if ((unsigned)li < (unsigned)BR && (unsigned)lk < (unsigned)BK)
+      atomicAdd(box + (comp * BR + li) * BK + lk, v);
     else
       g.put(comp, i, k, v);
   }
@@ -127,13 +129,13 @@ struct TileSinkT {
   }
 };
 using TileSink = TileSinkT<false>;
+using BunchSink = TileSinkT<true, 16, 32>;
 
-// block prologue: anchor the box at the cell of the block's first logical
-// particle (the smallest key of a sorted block) and clear it
-template <bool RHO>
-__device__ __forceinline__ void box_init(double *box, int anchor_i, int anchor_k, TileSinkT<RHO> &s,
-                                         const GlobalSink &g) {
-  for (int t = threadIdx.x; t < kBoxSize; t += blockDim.x) box[t] = 0.0;
+// block prologue: anchor the box one row and two columns below the given
+// cell (the block's smallest cell) and clear it
+template <class T>
+__device__ __forceinline__ void box_init(double *box, int anchor_i, int anchor_k, T &s, const GlobalSink &g) {
+  for (int t = threadIdx.x; t < T::kSize; t += blockDim.x) box[t] = 0.0;
   s.g = g;
   s.box = box;
   s.bi = anchor_i - 1;
@@ -146,16 +148,40 @@ __device__ __forceinline__ void box_init(double *box, int anchor_i, int anchor_k
 }
 
 // block epilogue: add the box to the grid (after __syncthreads)
-template <bool RHO>
-__device__ __forceinline__ void box_flush(const double *box, const TileSinkT<RHO> &s) {
-  for (int t = threadIdx.x; t < kBoxSize; t += blockDim.x) {
+template <class T>
+__device__ __forceinline__ void box_flush(const double *box, const T &s) {
+  for (int t = threadIdx.x; t < T::kSize; t += blockDim.x) {
     double v = box[t];
     if (v == 0.0) continue;
-    int comp = t / (kBoxR * kBoxK);
-    int rem = t - comp * kBoxR * kBoxK;
-    int li = rem / kBoxK, lk = rem - li * kBoxK;
+    int comp = t / (T::kR * T::kK);
+    int rem = t - comp * T::kR * T::kK;
+    int li = rem / T::kK, lk = rem - li * T::kK;
     s.g.put(comp, s.bi + li, s.bk + lk, v);
   }
 }
 
+}  // namespace pdp3
+
+namespace pdp3 {
+// smallest (i, k) over the block of two per-thread candidates (INT_MAX when
+// the thread saw no living particle); every thread gets the result
+__device__ __forceinline__ void block_min2(int &i, int &k) {
+  __shared__ int mi[8], mk[8];
+  for (int off = 32; off > 0; off >>= 1) {
+    i = min(i, __shfl_xor(i, off));
+    k = min(k, __shfl_xor(k, off));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    mi[wave] = i;
+    mk[wave] = k;
+  }
+  __syncthreads();
+  i = mi[0];
+  k = mk[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+    i = min(i, mi[w]);
+    k = min(k, mk[w]);
+  }
+}
 }  // namespace pdp3
