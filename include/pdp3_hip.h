@@ -186,6 +186,18 @@ int pdp3_sync(pdp3_ctx *ctx);
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
 
+/* Gauss-law residual R = div E - rho/eps0 on the grid nodes (no reference
+ * counterpart: the reference allocates c_rho_new for it but never fills it,
+ * src/loadInitParam.cpp:147,534).  rho is the charge density of every living
+ * particle of every species in the loop, weighted as
+ * Particles::charge_weighting (src/particles.cpp:101-226); div E is the
+ * discrete divergence that EField::calc_field (src/eField.cpp:85-170) leaves
+ * invariant, so R changes only by round-off when the deposit conserves charge
+ * (outside the PML conductivity layer and the bunch injection nodes).
+ * residual_out and rho_out (either may be NULL) receive n_grid_r x n_grid_z
+ * doubles; R is 0 at k = 0, k = n_grid_z-1 and i = n_grid_r-1. */
+int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out);
+
 /* live timing with HIP events on the context's stream (no reference
  * counterpart): enable, then read the accumulated device time in ms and the
  * launch count of one kernel class */

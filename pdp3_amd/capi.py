@@ -66,6 +66,7 @@ SIGNATURES = {
     "pdp3_sync": (_I, [_P]),
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
+    "pdp3_gauss_residual": (_I, [_P, _D, _D]),
     "pdp3_profile_enable": (_I, [_P, _I]),
     "pdp3_profile_get": (_I, [_P, _I, _D, C.POINTER(C.c_longlong)]),
     "pdp3_profile_reset": (_I, [_P]),
@@ -240,6 +241,14 @@ class Simulation:
 
     def charge_weighting(self, idx):
         self._chk(self.L.pdp3_particles_charge_weighting(self.h, idx))
+
+    def gauss_residual(self):
+        """(R, rho): R = div E - rho/eps0 on the nodes and the charge density of
+        all species, both (n_r, n_z)."""
+        R = np.empty((self.nr, self.nz))
+        rho = np.empty((self.nr, self.nz))
+        self._chk(self.L.pdp3_gauss_residual(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
+        return R, rho
 
     def sync(self):
         self._chk(self.L.pdp3_sync(self.h))

@@ -307,6 +307,8 @@ int pdp3_ctx_destroy(pdp3_ctx *ctx) {
   dfree(c.sort_keys);
   dfree(c.sort_perm);
   dfree(c.sort_ws);
+  dfree(c.rho_all);
+  dfree(c.gauss);
   if (c.err_host) (void)hipHostFree(c.err_host);
   for (auto &t : c.timers) {
     for (auto &ev : t.pending) {
@@ -750,6 +752,30 @@ int pdp3_particles_charge_weighting(pdp3_ctx *ctx, int index) {
   else
     launch_charge_weighting_f64(c, c.species[index]);
   CHECK_LAUNCH(ctx);
+  return PDP3_OK;
+}
+
+int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  int st;
+  if (!c.rho_all && (st = dalloc(ctx, &c.rho_all, c.cells()))) return st;
+  if (!c.gauss && (st = dalloc(ctx, &c.gauss, c.cells()))) return st;
+  launch_fill(c.rho_all, c.cells(), 0.0, c.stream);
+  // every species of the loop (bunches once they are created)
+  for (int j : c.order) {
+    if (is_f32(c))
+      launch_charge_weighting_f32(c, c.species[j], c.rho_all);
+    else
+      launch_charge_weighting_f64(c, c.species[j], c.rho_all);
+  }
+  launch_gauss_residual(c, c.rho_all, c.gauss);
+  CHECK_LAUNCH(ctx);
+  if ((st = check_device_errors(ctx))) return st;
+  const size_t b = c.cells() * sizeof(double);
+  if (residual_out) HIP_TRY(ctx, hipMemcpyAsync(residual_out, c.gauss, b, hipMemcpyDeviceToHost, c.stream));
+  if (rho_out) HIP_TRY(ctx, hipMemcpyAsync(rho_out, c.rho_all, b, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   return PDP3_OK;
 }
 

@@ -133,6 +133,9 @@ struct Ctx {
   size_t sort_ws_bytes = 0;
   void *sort_ws = nullptr;
   size_t sort_cap = 0;
+  // Gauss-law diagnostic: charge density of all species and the residual
+  double *rho_all = nullptr;
+  double *gauss = nullptr;
   size_t cells() const { return (size_t)g.nr * (size_t)g.nz; }
 };
 
@@ -140,18 +143,20 @@ struct Ctx {
 void launch_h_update(Ctx &c);
 void launch_e_update(Ctx &c);
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
+void launch_gauss_residual(Ctx &c, const double *rho, double *out);
 
 // FP64 plain form
 void launch_fused_f64(Ctx &c, Species &sp);
 void launch_step_v_f64(Ctx &c, Species &sp);
 void launch_full_current_f64(Ctx &c, Species &sp);
-void launch_charge_weighting_f64(Ctx &c, Species &sp);
+// rho: target grid (nullptr: the bunch density)
+void launch_charge_weighting_f64(Ctx &c, Species &sp, double *rho = nullptr);
 void launch_gather_f64(Ctx &c, int which, const double *r, const double *z, int n, double *out);
 void launch_bunch_inject_f64(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
 
 // FP32 form
 void launch_fused_f32(Ctx &c, Species &sp);
-void launch_charge_weighting_f32(Ctx &c, Species &sp);
+void launch_charge_weighting_f32(Ctx &c, Species &sp, double *rho = nullptr);
 void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
 bool launch_f32_from_f64(Ctx &c, Species &sp);  // pack absolute FP64 state into cell form
 void launch_f64_from_f32(Ctx &c, Species &sp);  // unpack for download

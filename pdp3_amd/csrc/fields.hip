@@ -97,6 +97,41 @@ __global__ void k_fill(double *p, size_t n, double v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+
+// Gauss-law residual R = div E - rho / eps0 on the nodes (r_i, z_k), the
+// points where Particles::charge_weighting puts rho (src/particles.cpp:101-226:
+// node k collects the cloud part between (k-1/2) dz and (k+1/2) dz, node i the
+// ring between (i-1/2) dr and (i+1/2) dr).  With E_r at (r_{i+1/2}, z_k) and
+// E_z at (r_i, z_{k+1/2}) (the staggering of src/eField.cpp:85-170), the
+// divergence the E update leaves invariant (div curl H = 0 exactly) is
+//   i >= 1: (r_{i+1/2} E_r[i] - r_{i-1/2} E_r[i-1]) / (r_i dr) + (E_z[k] - E_z[k-1]) / dz
+//   i = 0:  4 E_r[0] / dr + (E_z[k] - E_z[k-1]) / dz   (flux through the r = dr/2 disc)
+// so with a charge-conserving deposit R changes only by round-off (outside
+// the conductivity layer and the bunch injection nodes).  R = 0 on k = 0,
+// k = nz-1 and i = nr-1, where the divergence has no inner stencil.
+__global__ void __launch_bounds__(256) k_gauss_residual(GridDesc g, const double *__restrict__ er,
+                                                        const double *__restrict__ ez,
+                                                        const double *__restrict__ rho,
+                                                        double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (k >= g.nz) return;
+  const size_t p = at(g, i, k);
+  if (k == 0 || k == g.nz - 1 || i == g.nr - 1) {
+    out[p] = 0.0;
+    return;
+  }
+  double div;
+  const double dzt = (ez[p] - ez[p - 1]) / g.dz;
+  if (i == 0) {
+    div = 4.0 * er[p] / g.dr + dzt;
+  } else {
+    const double ri = (double)i;
+    div = ((ri + 0.5) * er[p] - (ri - 0.5) * er[p - g.nz]) / (ri * g.dr) + dzt;
+  }
+  out[p] = div - rho[p] / kEpsilon0;
+}
+
 }  // namespace
 
 void launch_h_update(Ctx &c) {
@@ -111,6 +146,11 @@ void launch_e_update(Ctx &c) {
   dim3 grid((c.g.nz - 1 + 255) / 256, c.g.nr - 1);
   hipLaunchKernelGGL(k_e_update, grid, block, 0, c.stream, c.g, c.a[kHR], c.a[kHPhi], c.a[kHZ], c.a[kJR],
                      c.a[kJPhi], c.a[kJZ], c.koef_e, c.koef_h, c.a[kER], c.a[kEPhi], c.a[kEZ]);
+}
+
+void launch_gauss_residual(Ctx &c, const double *rho, double *out) {
+  dim3 grid((c.g.nz + 255) / 256, c.g.nr);
+  hipLaunchKernelGGL(k_gauss_residual, grid, dim3(256), 0, c.stream, c.g, c.a[kER], c.a[kEZ], rho, out);
 }
 
 void launch_fill(double *p, size_t n, double v, hipStream_t s) {

@@ -664,10 +664,11 @@ void launch_fused_f32(Ctx &c, Species &sp) {
   }
 }
 
-void launch_charge_weighting_f32(Ctx &c, Species &sp) {
+void launch_charge_weighting_f32(Ctx &c, Species &sp, double *rho) {
   if (sp.n_cap <= 0 || !sp.f32.cell) return;
-  hipLaunchKernelGGL(k_charge32, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, args32(c, sp), sink32(c),
-                     sp.n_cap);
+  GlobalSink gs = sink32(c);
+  if (rho) gs.rho = rho;
+  hipLaunchKernelGGL(k_charge32, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, args32(c, sp), gs, sp.n_cap);
 }
 
 void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl) {

@@ -868,10 +868,11 @@ void launch_full_current_f64(Ctx &c, Species &sp) {
                      args_of(sp), sp.n_cap);
 }
 
-void launch_charge_weighting_f64(Ctx &c, Species &sp) {
+void launch_charge_weighting_f64(Ctx &c, Species &sp, double *rho) {
   if (sp.n_cap <= 0) return;
-  hipLaunchKernelGGL(k_charge, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, sink_of(c), args_of(sp),
-                     sp.n_cap);
+  GlobalSink gs = sink_of(c);
+  if (rho) gs.rho = rho;
+  hipLaunchKernelGGL(k_charge, dim3(blocks_for(sp.n_cap)), dim3(256), 0, c.stream, c.g, gs, args_of(sp), sp.n_cap);
 }
 
 void launch_gather_f64(Ctx &c, int which, const double *r, const double *z, int n, double *out) {

@@ -44,3 +44,43 @@ def compare_species(gpu_state, ora_state):
     for k in ("r", "z", "vr", "vphi", "vz"):
         out[k] = normwise(gpu_state[k][m], ora_state[k][m]) if m.any() else 0.0
     return out
+
+
+EPSILON0 = 8.85e-12  # include/constant.h:12
+
+
+def gauss_residual_np(er, ez, rho, dr, dz):
+    """div E - rho/eps0 on the nodes (r_i, z_k) where the charge weighting puts
+    rho, with E_r at (r_{i+1/2}, z_k) and E_z at (r_i, z_{k+1/2}): the
+    divergence written out from the Gauss theorem on the node's ring cell
+    (radial faces at r_{i-1/2}, r_{i+1/2}; the axis cell is the disc of radius
+    dr/2), independently of the device kernel.  0 on k = 0, k = nz-1 and
+    i = nr-1."""
+    nr, nz = er.shape
+    R = np.zeros((nr, nz))
+    for i in range(nr - 1):
+        if i == 0:
+            area_out = 2 * np.pi * (dr / 2) * dz        # side of the disc
+            vol = np.pi * (dr / 2) ** 2 * dz
+            flux_r = area_out * er[0, 1:-1]
+            cap = np.pi * (dr / 2) ** 2                  # top / bottom of the disc
+        else:
+            a_out = 2 * np.pi * (i + 0.5) * dr * dz
+            a_in = 2 * np.pi * (i - 0.5) * dr * dz
+            vol = 2 * np.pi * i * dr * dr * dz
+            flux_r = a_out * er[i, 1:-1] - a_in * er[i - 1, 1:-1]
+            cap = 2 * np.pi * i * dr * dr
+        flux_z = cap * (ez[i, 1:-1] - ez[i, :-2])
+        R[i, 1:-1] = (flux_r + flux_z) / vol - rho[i, 1:-1] / EPSILON0
+    return R
+
+
+def oracle_rho_all(orc, sim):
+    """charge density of every species of an oracle Sim (the reference's
+    charge weighting, src/particles.cpp:101-226)"""
+    import ctypes as C
+    out = np.zeros((sim.nr, sim.nz))
+    for idx in sim.species:
+        st = orc.lib().or_sim_charge_weighting(sim.h, idx, out.ctypes.data_as(C.POINTER(C.c_double)))
+        assert st == 0
+    return out
