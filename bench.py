@@ -118,7 +118,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg3")
-    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--sort-interval", type=int, default=20,
+                    help="cell sort of the plasma every N steps (counted in the timed steps)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-form comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",

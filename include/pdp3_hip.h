@@ -198,6 +198,17 @@ int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
  * doubles; R is 0 at k = 0, k = n_grid_z-1 and i = n_grid_r-1. */
 int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out);
 
+/* tile shape of the FP32 particle kernel (no reference counterpart):
+ * particles walked per lane (ppt, 4 / 8 / 16) and wavefronts per block
+ * (waves); supported pairs: (16,4) (16,2) (16,1) (8,4) (8,2) (4,4).  The
+ * FP64 form has one shape.  pdp3_autotune runs steps_per_candidate real steps
+ * of the simulation with each shape, times the plasma kernel and keeps the
+ * fastest (the steps advance the simulation; results differ between shapes
+ * only by the order of FP64 sums). */
+int pdp3_set_tile(pdp3_ctx *ctx, int ppt, int waves);
+int pdp3_get_tile(const pdp3_ctx *ctx, int *ppt, int *waves);
+int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves);
+
 /* live timing with HIP events on the context's stream (no reference
  * counterpart): enable, then read the accumulated device time in ms and the
  * launch count of one kernel class */

@@ -67,6 +67,9 @@ SIGNATURES = {
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
+    "pdp3_set_tile": (_I, [_P, _I, _I]),
+    "pdp3_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "pdp3_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
     "pdp3_profile_enable": (_I, [_P, _I]),
     "pdp3_profile_get": (_I, [_P, _I, _D, C.POINTER(C.c_longlong)]),
     "pdp3_profile_reset": (_I, [_P]),
@@ -249,6 +252,20 @@ class Simulation:
         rho = np.empty((self.nr, self.nz))
         self._chk(self.L.pdp3_gauss_residual(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
         return R, rho
+
+    def set_tile(self, ppt, waves):
+        self._chk(self.L.pdp3_set_tile(self.h, int(ppt), int(waves)))
+
+    def tile(self):
+        a, b = _I(), _I()
+        self._chk(self.L.pdp3_get_tile(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, steps_per_candidate=2):
+        """Times real steps with every tile shape, keeps the fastest; returns (ppt, waves)."""
+        a, b = _I(), _I()
+        self._chk(self.L.pdp3_autotune(self.h, int(steps_per_candidate), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def sync(self):
         self._chk(self.L.pdp3_sync(self.h))

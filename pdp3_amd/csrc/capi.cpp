@@ -830,6 +830,54 @@ int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval) {
   return PDP3_OK;
 }
 
+int pdp3_set_tile(pdp3_ctx *ctx, int ppt, int waves) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  for (int t = 0; t < kNumTiles; t++)
+    if (kTileShapes[t][0] == ppt && kTileShapes[t][1] == waves) {
+      ctx->c.tile = t;
+      return PDP3_OK;
+    }
+  return fail(ctx, PDP3_ERR_BAD_ARG, "unsupported tile shape");
+}
+
+int pdp3_get_tile(const pdp3_ctx *ctx, int *ppt, int *waves) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  if (ppt) *ppt = kTileShapes[ctx->c.tile][0];
+  if (waves) *waves = kTileShapes[ctx->c.tile][1];
+  return PDP3_OK;
+}
+
+int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves) {
+  if (!ctx || steps_per_candidate < 1) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (!is_f32(c)) return pdp3_get_tile(ctx, ppt, waves);  // one tile shape in the FP64 form
+  // each candidate times real steps of the simulation on the plasma kernel
+  const bool was_profiling = c.profiling;
+  KernelTimer saved = c.timers[PDP3_KERNEL_PARTICLES];
+  int st, best = 0;
+  double best_ms = 1e300;
+  for (int t = 0; t < kNumTiles; t++) {
+    c.tile = t;
+    c.profiling = true;
+    if ((st = check_device_errors(ctx))) break;
+    c.timers[PDP3_KERNEL_PARTICLES].total_ms = 0;
+    c.timers[PDP3_KERNEL_PARTICLES].launches = 0;
+    if ((st = pdp3_step(ctx, steps_per_candidate))) break;
+    if ((st = check_device_errors(ctx))) break;
+    const double ms = c.timers[PDP3_KERNEL_PARTICLES].total_ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  c.tile = best;
+  c.profiling = was_profiling;
+  c.timers[PDP3_KERNEL_PARTICLES].total_ms = saved.total_ms;
+  c.timers[PDP3_KERNEL_PARTICLES].launches = saved.launches;
+  if (st) return st;
+  return pdp3_get_tile(ctx, ppt, waves);
+}
+
 int pdp3_profile_enable(pdp3_ctx *ctx, int enable) {
   if (!ctx) return PDP3_ERR_BAD_ARG;
   ctx->c.profiling = enable != 0;

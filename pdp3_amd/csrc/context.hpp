@@ -59,6 +59,11 @@ __host__ __device__ __forceinline__ long long logical_of(long long p) {
   return c * kChunk + (w % 64) * kPPT + w / 64;
 }
 
+// tile shapes of the FP32 particle kernel: particles per lane (PPT) and waves
+// per block; the autotuner picks one per context
+constexpr int kNumTiles = 6;
+constexpr int kTileShapes[kNumTiles][2] = {{16, 4}, {16, 2}, {16, 1}, {8, 4}, {8, 2}, {4, 4}};
+
 // alive byte of the FP64 state: 0 dead, 1 alive, 2 padding slot
 constexpr uint8_t kPadSlot = 2;
 // FP32 cell word of a padding slot (a dead particle's word never equals it)
@@ -126,6 +131,8 @@ struct Ctx {
   bool profiling = false;
   KernelTimer timers[5];
   int sort_interval = 0;
+  // tile shape of the FP32 particle kernel (index into kTileShapes)
+  int tile = 0;
   long long steps_done = 0;
   // sort workspace
   int32_t *sort_keys = nullptr;

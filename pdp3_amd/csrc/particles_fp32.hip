@@ -126,6 +126,86 @@ __device__ __forceinline__ float gather(const F32Args &p, const double *__restri
   return (f00 * ob + f01 * b) * rw.A + (f10 * ob + f11 * b) * rw.B;
 }
 
+// field sources of the push.  GlobalFields reads the six FP64 components
+// from HBM / L2 per particle; BoxFields serves the particles of a cell-sorted
+// block from an LDS copy (converted to FP32 once per node) of the nodes their
+// cells can touch, and falls back to HBM for the rest.  Both evaluate the same
+// FP32 expression on the same FP32-rounded node values, so the result does not
+// depend on the source.
+struct GlobalFields {
+  __device__ __forceinline__ void get(const F32Args &p, int, int, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
+                                      float h[3]) const {
+    e[0] = gather(p, p.er, rl, zo);
+    e[1] = gather(p, p.ephi, ro, zo);
+    e[2] = gather(p, p.ez, ro, zh);
+    h[0] = gather(p, p.hhr, ro, zh);
+    h[1] = gather(p, p.hhphi, rl, zh);
+    h[2] = gather(p, p.hhz, rl, zo);
+  }
+};
+
+// rows bi..bi+kFR-1, columns bk..bk+kFK-1 of the six components, anchored one
+// row and one column below the block's first cell (ai, ak): a particle in cell
+// (i, k) reads rows i-1..i+1 and columns k-1..k+1, so every cell with
+// i in [ai, ai+1] and k in [ak, ak+kFK-3] is served from the box
+constexpr int kFR = 4, kFK = 64, kFComp = 6, kFSize = kFComp * kFR * kFK;
+
+__device__ __forceinline__ float gather_box(const float *F, RadW rw, AxW aw, int bi, int bk) {
+  const float *row0 = F + (rw.ir - bi) * kFK + (aw.kz - bk);
+  float f00 = row0[0], f01 = row0[1], f10 = row0[kFK], f11 = row0[kFK + 1];
+  float b = aw.b, ob = 1.f - aw.b;
+  return (f00 * ob + f01 * b) * rw.A + (f10 * ob + f11 * b) * rw.B;
+}
+
+struct BoxFields {
+  const float *box;  // LDS, kFComp x kFR x kFK
+  int ai, ak;        // first cell of the block
+  __device__ __forceinline__ void get(const F32Args &p, int i, int k, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
+                                      float h[3]) const {
+    const bool inbox =
+        (unsigned)(i - ai) < 2u && (unsigned)(k - ak) < (unsigned)(kFK - 2) && i <= p.nr - 2 && k <= p.nz - 2;
+    if (inbox) {
+      const int bi = ai - 1, bk = ak - 1;
+      constexpr int C = kFR * kFK;
+      const float e0 = gather_box(box + 0 * C, rl, zo, bi, bk);
+      const float e1 = gather_box(box + 1 * C, ro, zo, bi, bk);
+      const float e2 = gather_box(box + 2 * C, ro, zh, bi, bk);
+      const float h0 = gather_box(box + 3 * C, ro, zh, bi, bk);
+      const float h1 = gather_box(box + 4 * C, rl, zh, bi, bk);
+      const float h2 = gather_box(box + 5 * C, rl, zo, bi, bk);
+      e[0] = e0; e[1] = e1; e[2] = e2;
+      h[0] = h0; h[1] = h1; h[2] = h2;
+    } else {
+      const float e0 = gather(p, p.er, rl, zo);
+      const float e1 = gather(p, p.ephi, ro, zo);
+      const float e2 = gather(p, p.ez, ro, zh);
+      const float h0 = gather(p, p.hhr, ro, zh);
+      const float h1 = gather(p, p.hhphi, rl, zh);
+      const float h2 = gather(p, p.hhz, rl, zo);
+      e[0] = e0; e[1] = e1; e[2] = e2;
+      h[0] = h0; h[1] = h1; h[2] = h2;
+    }
+  }
+};
+
+// block prologue: fill the field box (nodes outside the grid read as 0; no
+// particle that the box serves reads them)
+__device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int ai, int ak) {
+  const int bi = ai - 1, bk = ak - 1;
+  const double *src[kFComp] = {p.er, p.ephi, p.ez, p.hhr, p.hhphi, p.hhz};
+#pragma unroll
+  for (int comp = 0; comp < kFComp; comp++) {
+    const double *F = src[comp];
+    for (int t = threadIdx.x; t < kFR * kFK; t += blockDim.x) {
+      const int r = t / kFK, col = t - r * kFK;
+      const int gi = bi + r, gk = bk + col;
+      float v = 0.f;
+      if ((unsigned)gi < (unsigned)p.nr && (unsigned)gk < (unsigned)p.nz) v = (float)F[(size_t)gi * p.nz + gk];
+      box[comp * kFR * kFK + t] = v;
+    }
+  }
+}
+
 __device__ __forceinline__ float dot3f(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 // relativistic Boris on u = gamma v, src/particles.cpp:1025-1150
@@ -376,8 +456,8 @@ __device__ __forceinline__ void walls(const F32Args &p, int &i, float &xa, int &
 }
 
 // the whole step of one particle in slot n
-template <class S>
-__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, long long n) {
+template <class S, class FS>
+__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, long long n) {
   uint32_t c = p.cell[n];
   if (c & kDead) return;
   int i = cell_i(c), k = cell_k(c);
@@ -389,12 +469,7 @@ __device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, long lo
   RadW rl = rad_lo(i, a), ro = rad_own(i, a);
   AxW zo = ax_own(k, b), zh = ax_half(k, b);
   float e[3], h[3];
-  e[0] = gather(p, p.er, rl, zo);
-  e[1] = gather(p, p.ephi, ro, zo);
-  e[2] = gather(p, p.ez, ro, zh);
-  h[0] = gather(p, p.hhr, ro, zh);
-  h[1] = gather(p, p.hhphi, rl, zh);
-  h[2] = gather(p, p.hhz, rl, zo);
+  fs.get(p, i, k, rl, ro, zo, zh, e, h);
   if (p.pusher == 1)
     vay_u(p, e, h, u);
   else
@@ -435,9 +510,15 @@ __device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, long lo
 
 // chunked plasma species: 4 wavefronts per block, each walking one chunk of
 // kChunk slots kPPT particles per lane, deposits through the tile sink
-__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
-  __shared__ double box[TileSink::kSize];
-  const long long base = (long long)blockIdx.x * kBlockParticles;
+// one tile of a cell-sorted (chunked) species: CB = WAVES * PPT / kPPT chunks
+// starting at slot `base`.  Wave w takes chunk w / (kPPT / PPT) and, in it,
+// the slots of iterations j0 .. j0+PPT-1, j0 = (w % (kPPT / PPT)) * PPT: its
+// lanes read contiguous slots (coalesced) and lane L walks the consecutive
+// logical particles L*kPPT+j0 .. L*kPPT+j0+PPT-1, mostly of one cell.
+template <int PPT, int WAVES>
+__device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink &gs, long long base, long long n_cap,
+                                             double *box, float *fbox) {
+  static_assert(kPPT % PPT == 0 && (WAVES * PPT) % kPPT == 0, "tile shape");
   int ai = 0, ak = 0;
   if (base < n_cap) {
     uint32_t c0 = p.cell[base];
@@ -448,16 +529,28 @@ __global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs,
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);
+  field_box_fill(p, fbox, ai, ak);
   __syncthreads();
+  const BoxFields fs{fbox, ai, ak};
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long cbase = base + (long long)wave * kChunk + lane;
-  for (int j = 0; j < kPPT; j++) {
+  constexpr int kSplit = kPPT / PPT;
+  const int chunk = wave / kSplit, j0 = (wave % kSplit) * PPT;
+  const long long cbase = base + (long long)chunk * kChunk + (long long)j0 * 64 + lane;
+  for (int j = 0; j < PPT; j++) {
     long long n = cbase + (long long)j * 64;
-    if (n < n_cap) step_particle32(p, sk, n);
+    if (n < n_cap) step_particle32(p, sk, fs, n);
   }
   sk.flush();
   __syncthreads();
   box_flush(box, sk);
+}
+
+template <int PPT, int WAVES>
+__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
+  __shared__ double box[TileSink::kSize];
+  __shared__ float fbox[kFSize];
+  constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
+  fused32_tile<PPT, WAVES>(p, gs, (long long)blockIdx.x * kTile, n_cap, box, fbox);
 }
 
 // bunches (plain layout): the same walk, the box anchored at the smallest
@@ -485,7 +578,7 @@ __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs,
   __syncthreads();
   for (int j = 0; j < kPPT; j++) {
     long long n = cbase + (long long)j * 64;
-    if (n < n_cap) step_particle32(p, sk, n);
+    if (n < n_cap) step_particle32(p, sk, GlobalFields(), n);
   }
   sk.flush();
   __syncthreads();
@@ -497,7 +590,7 @@ __global__ void __launch_bounds__(256) k_fused32_plain(F32Args p, GlobalSink gs,
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_cap) return;
   GlobalSink sk = gs;
-  step_particle32(p, sk, n);
+  step_particle32(p, sk, GlobalFields(), n);
 }
 
 __global__ void k_charge32(F32Args p, GlobalSink gs, long long n_cap) {
@@ -653,11 +746,28 @@ unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+template <int PPT, int WAVES>
+void launch_tile_t(hipStream_t st, const F32Args &p, const GlobalSink &gs, long long n_cap) {
+  constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
+  unsigned blocks = (unsigned)((n_cap + kTile - 1) / kTile);
+  hipLaunchKernelGGL((k_fused32_tiled<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, p, gs, n_cap);
+}
+
+void launch_tile(int tile, hipStream_t st, const F32Args &p, const GlobalSink &gs, long long n_cap) {
+  switch (tile) {
+    case 0: launch_tile_t<16, 4>(st, p, gs, n_cap); break;
+    case 1: launch_tile_t<16, 2>(st, p, gs, n_cap); break;
+    case 2: launch_tile_t<16, 1>(st, p, gs, n_cap); break;
+    case 3: launch_tile_t<8, 4>(st, p, gs, n_cap); break;
+    case 4: launch_tile_t<8, 2>(st, p, gs, n_cap); break;
+    default: launch_tile_t<4, 4>(st, p, gs, n_cap); break;
+  }
+}
+
 void launch_fused_f32(Ctx &c, Species &sp) {
   if (sp.n_cap <= 0 || !sp.f32.cell) return;
   if (sp.chunked) {
-    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
-    hipLaunchKernelGGL(k_fused32_tiled, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);
+    launch_tile(c.tile, c.stream, args32(c, sp), sink32(c), sp.n_cap);
   } else {
     unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
     hipLaunchKernelGGL(k_fused32_bunch, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);

@@ -95,3 +95,43 @@ def test_cfg1_100_steps_error(pusher, orc):
     print("relative RMS errors after 100 steps", errs)
     for name in ("E_r", "E_z", "H_phi"):
         assert errs[name] <= 1e-3, (name, errs[name])
+
+
+def test_tile_shapes_agree_and_autotune():
+    """Every tile shape of the FP32 plasma kernel computes the same step (the
+    per-particle FP32 arithmetic is identical; only the order of the FP64 grid
+    sums differs); the autotuner returns one of them."""
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    w = wl.config("cfg1", pml_cells=8)
+    w.bunches = []
+    shapes = [(16, 4), (16, 2), (16, 1), (8, 4), (8, 2), (4, 4)]
+    ref = None
+    for ppt, waves in shapes:
+        print("tile", ppt, waves, flush=True)
+        sim = wl.build_simulation(w, precision=capi.PREC_FP32)
+        sim.set_tile(ppt, waves)
+        assert sim.tile() == (ppt, waves)
+        sim.step(10)
+        f = {n: sim.field(n) for n in ("E_r", "E_z", "H_phi", "J_z")}
+        st = sim.download(0)
+        sim.close()
+        if ref is None:
+            ref = (f, st)
+            continue
+        for n in f:
+            assert normwise(f[n], ref[0][n]) <= 1e-10, (ppt, waves, n, normwise(f[n], ref[0][n]))
+        for k in ("r", "z", "vr", "vz"):
+            assert normwise(st[k], ref[1][k]) <= 1e-10, (ppt, waves, k)
+    sim = wl.build_simulation(w, precision=capi.PREC_FP32)
+    best = sim.autotune(1)
+    assert best in shapes
+    assert sim.time > 0
+    sim.close()
+    with pytest.raises(capi.Pdp3Error):
+        sim2 = wl.build_simulation(w, precision=capi.PREC_FP32)
+        try:
+            sim2.set_tile(3, 3)
+        finally:
+            sim2.close()
