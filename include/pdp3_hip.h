@@ -141,6 +141,14 @@ int pdp3_bunch_add(pdp3_ctx *ctx, int charge_e, double mass_e, int number,
                    const double *r, const double *w, const double *inject_u,
                    double duration, double hole_duration, double velocity,
                    int bunch_number, int *out_index);
+/* a bunch that is already inside the grid (no reference counterpart: the
+ * reference's bunches enter by injection only): alive particles at the given
+ * positions and velocities, absorbed at r_max / z_max like Bunch
+ * (src/bunch.cpp:126-153), charge-weighted into the bunch density every
+ * step; it joins the species list at once. */
+int pdp3_bunch_add_loaded(pdp3_ctx *ctx, int charge_e, double mass_e, int number, const double *r,
+                          const double *z, const double *vr, const double *vphi, const double *vz,
+                          const double *w, int *out_index);
 /* upload / download of a species' state.  w = real particles per
  * macro-particle (charge_array = charge * w, src/particles.cpp:328-329);
  * alive as Particles::is_alive.  Any pointer may be NULL (kept / skipped). */
@@ -208,6 +216,29 @@ int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out);
 int pdp3_set_tile(pdp3_ctx *ctx, int ppt, int waves);
 int pdp3_get_tile(const pdp3_ctx *ctx, int *ppt, int *waves);
 int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves);
+
+/* ---- ensembles (no reference counterpart) ----
+ * Independent instances (contexts set up with the calls above: any grid,
+ * PML, plasma, loaded bunches; FP32 form, one common delta_t) advanced
+ * together: per step one graph replay with an H launch, a J / rho reset, ONE
+ * persistent particle launch over the plasma tiles and bunch blocks of every
+ * instance, and an E launch.  The contexts stay owned by the caller and must
+ * not be stepped on their own while the ensemble lives; their clocks advance
+ * with the ensemble.  Bunches must be loaded (pdp3_bunch_add_loaded). */
+typedef struct pdp3_ensemble pdp3_ensemble;
+int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out);
+int pdp3_ensemble_destroy(pdp3_ensemble *e);
+const char *pdp3_ensemble_last_error(const pdp3_ensemble *e);
+int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps);
+/* cell sort of every instance every sort_interval steps (outside the graph) */
+int pdp3_ensemble_set_sort_interval(pdp3_ensemble *e, int sort_interval);
+/* device time of the stepped graphs since the last call (HIP events), and the step count */
+int pdp3_ensemble_timing(pdp3_ensemble *e, double *total_ms, long long *steps);
+/* tile shape of the persistent particle kernel, as pdp3_set_tile; the
+ * autotuner times steps_per_candidate real steps per shape and keeps the fastest */
+int pdp3_ensemble_set_tile(pdp3_ensemble *e, int ppt, int waves);
+int pdp3_ensemble_get_tile(const pdp3_ensemble *e, int *ppt, int *waves);
+int pdp3_ensemble_autotune(pdp3_ensemble *e, int steps_per_candidate, int *ppt, int *waves);
 
 /* live timing with HIP events on the context's stream (no reference
  * counterpart): enable, then read the accumulated device time in ms and the

@@ -67,6 +67,16 @@ SIGNATURES = {
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
+    "pdp3_bunch_add_loaded": (_I, [_P, _I, _F, _I, _D, _D, _D, _D, _D, _D, C.POINTER(_I)]),
+    "pdp3_ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
+    "pdp3_ensemble_destroy": (_I, [_P]),
+    "pdp3_ensemble_last_error": (C.c_char_p, [_P]),
+    "pdp3_ensemble_step": (_I, [_P, _I]),
+    "pdp3_ensemble_set_sort_interval": (_I, [_P, _I]),
+    "pdp3_ensemble_timing": (_I, [_P, _D, C.POINTER(C.c_longlong)]),
+    "pdp3_ensemble_set_tile": (_I, [_P, _I, _I]),
+    "pdp3_ensemble_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "pdp3_ensemble_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
     "pdp3_set_tile": (_I, [_P, _I, _I]),
     "pdp3_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "pdp3_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
@@ -200,6 +210,14 @@ class Simulation:
                     alive=np.ones(len(r), np.uint8) if alive is None else alive)
         return idx
 
+    def add_bunch_loaded(self, charge_e, mass_e, r, z, vr, vphi, vz, w):
+        """A bunch already inside the grid (alive, no injection)."""
+        arrs = [_f64(a) for a in (r, z, vr, vphi, vz, w)]
+        idx = _I()
+        self._chk(self.L.pdp3_bunch_add_loaded(self.h, int(charge_e), float(mass_e), len(arrs[0]),
+                                               *(_dp(a) for a in arrs), C.byref(idx)))
+        return idx.value
+
     def add_bunch(self, charge_e, mass_e, r, w, inject_u, duration, hole_duration, velocity, bunch_number=0):
         r, w, u = _f64(r), _f64(w), _f64(inject_u)
         idx = _I()
@@ -287,3 +305,59 @@ class Simulation:
         ms, n = C.c_double(), C.c_longlong()
         self._chk(self.L.pdp3_profile_get(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Ensemble:
+    """Independent FP32-form simulations advanced together (one graph replay
+    per step with one persistent particle launch over every instance)."""
+
+    def __init__(self, sims):
+        self.L = lib()
+        self.sims = list(sims)
+        arr = (_P * len(self.sims))(*[s.h for s in self.sims])
+        h = _P()
+        st = self.L.pdp3_ensemble_create(arr, len(self.sims), C.byref(h))
+        if st != 0:
+            raise Pdp3Error(st, "pdp3_ensemble_create failed (FP32 contexts, one dt, loaded bunches only)")
+        self.h = h
+
+    def _chk(self, st):
+        if st != 0:
+            raise Pdp3Error(st, self.L.pdp3_ensemble_last_error(self.h).decode())
+
+    def step(self, n=1):
+        self._chk(self.L.pdp3_ensemble_step(self.h, int(n)))
+
+    def set_sort_interval(self, n):
+        self._chk(self.L.pdp3_ensemble_set_sort_interval(self.h, int(n)))
+
+    def timing(self):
+        """(device ms of the stepped graphs, steps) since the last call"""
+        ms, n = C.c_double(), C.c_longlong()
+        self._chk(self.L.pdp3_ensemble_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_tile(self, ppt, waves):
+        self._chk(self.L.pdp3_ensemble_set_tile(self.h, int(ppt), int(waves)))
+
+    def tile(self):
+        a, b = _I(), _I()
+        self._chk(self.L.pdp3_ensemble_get_tile(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, steps_per_candidate=2):
+        a, b = _I(), _I()
+        self._chk(self.L.pdp3_ensemble_autotune(self.h, int(steps_per_candidate), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pdp3_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

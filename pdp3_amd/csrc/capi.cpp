@@ -622,6 +622,46 @@ int pdp3_species_count(const pdp3_ctx *ctx, int index, int *number) {
   return PDP3_OK;
 }
 
+int pdp3_bunch_add_loaded(pdp3_ctx *ctx, int charge_e, double mass_e, int number, const double *r,
+                          const double *z, const double *vr, const double *vphi, const double *vz, const double *w,
+                          int *out_index) {
+  if (!ctx || number < 0 || !(mass_e > 0) || (number > 0 && (!r || !z || !vr || !vphi || !vz || !w)))
+    return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  Species sp;
+  sp.kind = PDP3_SPECIES_BUNCH;
+  sp.charge_e = charge_e;
+  sp.mass_e = mass_e;
+  sp.charge = charge_e * kElCharge;
+  sp.mass = mass_e * kElMass;
+  sp.n = number;
+  sp.duration = 0;  // nothing left to inject
+  sp.created = true;
+  sp.chunked = false;
+  int st = alloc_species_state(ctx, sp);
+  if (st) {
+    free_species(sp);
+    return st;
+  }
+  const size_t b = sizeof(double) * (size_t)number;
+  if (number > 0) {
+    const double *src[6] = {r, z, vr, vphi, vz, w};
+    double *dst[6] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w};
+    for (int j = 0; j < 6; j++) HIP_TRY(ctx, hipMemcpyAsync(dst[j], src[j], b, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipMemsetAsync(sp.f64.alive, 1, (size_t)number, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  c.species.push_back(sp);
+  const int idx = (int)c.species.size() - 1;
+  c.order.push_back(idx);
+  if (is_f32(c)) {
+    if (!launch_f32_from_f64(c, c.species[idx])) return fail(ctx, PDP3_ERR_OOM, "FP32 species arrays");
+    CHECK_LAUNCH(ctx);
+  }
+  if (out_index) *out_index = idx;
+  return PDP3_OK;
+}
+
 int pdp3_species_upload(pdp3_ctx *ctx, int index, const double *r, const double *z, const double *vr,
                         const double *vphi, const double *vz, const double *w, const uint8_t *alive) {
   if (!valid_species(ctx, index)) return PDP3_ERR_BAD_ARG;
@@ -876,6 +916,229 @@ int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves) 
   c.timers[PDP3_KERNEL_PARTICLES].launches = saved.launches;
   if (st) return st;
   return pdp3_get_tile(ctx, ppt, waves);
+}
+
+// ---------------------------------------------------------------- ensembles
+
+}  // extern "C"
+
+struct pdp3_ensemble {
+  std::vector<pdp3_ctx *> ctxs;
+  std::vector<Ctx *> cs;
+  hipStream_t stream = nullptr;
+  EnsFields *fields = nullptr;
+  EnsParticles *parts = nullptr;
+  int tile = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_tile = -1;
+  int sort_interval = 0;
+  long long steps_done = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double timed_ms = 0;
+  long long timed_steps = 0;
+  std::string last_error;
+};
+
+namespace {
+
+int efail(pdp3_ensemble *e, int code, const std::string &msg) {
+  e->last_error = msg;
+  return code;
+}
+
+#define ENS_HIP(e, expr)                                                         \
+  do {                                                                           \
+    hipError_t _err = (expr);                                                    \
+    if (_err != hipSuccess) return efail((e), PDP3_ERR_HIP, hipGetErrorString(_err)); \
+  } while (0)
+
+void ens_drop_graph(pdp3_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_tile = -1;
+}
+
+// one loop iteration of every instance (src/loadInitParam.cpp:471-587 without
+// init_beam: ensemble bunches are loaded, not injected), captured once per
+// tile shape and replayed
+int ens_graph(pdp3_ensemble *e) {
+  if (e->exec && e->graph_tile == e->tile) return PDP3_OK;
+  ens_drop_graph(e);
+  ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  ens_launch_h(e->fields, e->stream);
+  ens_launch_zero(e->fields, e->stream);
+  ens_launch_particles(e->parts, e->tile, e->stream);
+  ens_launch_e(e->fields, e->stream);
+  hipGraph_t g = nullptr;
+  ENS_HIP(e, hipStreamEndCapture(e->stream, &g));
+  e->graph = g;
+  ENS_HIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  e->graph_tile = e->tile;
+  return PDP3_OK;
+}
+
+int ens_check(pdp3_ensemble *e) {
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  for (pdp3_ctx *ctx : e->ctxs) {
+    int st = check_device_errors(ctx);
+    if (st) return efail(e, st, std::string("instance: ") + ctx->c.last_error);
+  }
+  return PDP3_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out) {
+  if (!ctxs || n < 1 || !out) return PDP3_ERR_BAD_ARG;
+  *out = nullptr;
+  auto *e = new pdp3_ensemble;
+  for (int j = 0; j < n; j++) {
+    pdp3_ctx *ctx = ctxs[j];
+    if (!ctx) {
+      delete e;
+      return PDP3_ERR_BAD_ARG;
+    }
+    Ctx &c = ctx->c;
+    bool ok = is_f32(c) && c.g.dt == ctxs[0]->c.g.dt;
+    for (const Species &sp : c.species)
+      if (sp.kind == PDP3_SPECIES_BUNCH && !(sp.created && sp.duration == 0)) ok = false;  // loaded bunches only
+    if (!ok) {
+      delete e;
+      return PDP3_ERR_BAD_ARG;
+    }
+    if (hipStreamSynchronize(c.stream) != hipSuccess) {
+      delete e;
+      return PDP3_ERR_HIP;
+    }
+    e->ctxs.push_back(ctx);
+    e->cs.push_back(&c);
+  }
+  if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
+      hipEventCreate(&e->ev1) != hipSuccess) {
+    pdp3_ensemble_destroy(e);
+    return PDP3_ERR_HIP;
+  }
+  e->fields = ens_fields_build(e->cs);
+  e->parts = e->fields ? ens_particles_build(e->cs) : nullptr;
+  if (!e->fields || !e->parts) {
+    pdp3_ensemble_destroy(e);
+    return PDP3_ERR_OOM;
+  }
+  *out = e;
+  return PDP3_OK;
+}
+
+int pdp3_ensemble_destroy(pdp3_ensemble *e) {
+  if (!e) return PDP3_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_drop_graph(e);
+  ens_fields_free(e->fields);
+  ens_particles_free(e->parts);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return PDP3_OK;
+}
+
+const char *pdp3_ensemble_last_error(const pdp3_ensemble *e) { return e ? e->last_error.c_str() : ""; }
+
+int pdp3_ensemble_set_sort_interval(pdp3_ensemble *e, int sort_interval) {
+  if (!e || sort_interval < 0) return PDP3_ERR_BAD_ARG;
+  e->sort_interval = sort_interval;
+  return PDP3_OK;
+}
+
+int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps) {
+  if (!e || nsteps < 0) return PDP3_ERR_BAD_ARG;
+  int st;
+  bool timing = false;
+  for (int s = 0; s < nsteps; s++) {
+    if (e->sort_interval > 0 && e->steps_done > 0 && e->steps_done % e->sort_interval == 0) {
+      ENS_HIP(e, hipStreamSynchronize(e->stream));
+      for (pdp3_ctx *ctx : e->ctxs) {
+        if ((st = pdp3_sort_particles(ctx)) || (st = pdp3_sync(ctx))) return efail(e, st, ctx->c.last_error);
+      }
+      if (ens_particles_update(e->parts, e->cs) != 0) return efail(e, PDP3_ERR_HIP, "ensemble tables");
+    }
+    if ((st = ens_graph(e))) return st;
+    if (!timing) {
+      ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
+      timing = true;
+    }
+    ENS_HIP(e, hipGraphLaunch(e->exec, e->stream));
+    for (Ctx *c : e->cs) {
+      c->current_time = c->current_time + c->g.dt;  // src/loadInitParam.cpp:586
+      c->steps_done++;
+    }
+    e->steps_done++;
+    // close the timed interval before a sort (which runs outside the graph)
+    const bool sort_next = e->sort_interval > 0 && e->steps_done % e->sort_interval == 0;
+    if (timing && (sort_next || s == nsteps - 1)) {
+      ENS_HIP(e, hipEventRecord(e->ev1, e->stream));
+      ENS_HIP(e, hipEventSynchronize(e->ev1));
+      float ms = 0;
+      ENS_HIP(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+      e->timed_ms += ms;
+      timing = false;
+    }
+  }
+  e->timed_steps += nsteps;
+  return ens_check(e);
+}
+
+int pdp3_ensemble_timing(pdp3_ensemble *e, double *total_ms, long long *steps) {
+  if (!e) return PDP3_ERR_BAD_ARG;
+  if (total_ms) *total_ms = e->timed_ms;
+  if (steps) *steps = e->timed_steps;
+  e->timed_ms = 0;
+  e->timed_steps = 0;
+  return PDP3_OK;
+}
+
+int pdp3_ensemble_set_tile(pdp3_ensemble *e, int ppt, int waves) {
+  if (!e) return PDP3_ERR_BAD_ARG;
+  for (int t = 0; t < kNumTiles; t++)
+    if (kTileShapes[t][0] == ppt && kTileShapes[t][1] == waves) {
+      e->tile = t;
+      return PDP3_OK;
+    }
+  return efail(e, PDP3_ERR_BAD_ARG, "unsupported tile shape");
+}
+
+int pdp3_ensemble_get_tile(const pdp3_ensemble *e, int *ppt, int *waves) {
+  if (!e) return PDP3_ERR_BAD_ARG;
+  if (ppt) *ppt = kTileShapes[e->tile][0];
+  if (waves) *waves = kTileShapes[e->tile][1];
+  return PDP3_OK;
+}
+
+int pdp3_ensemble_autotune(pdp3_ensemble *e, int steps_per_candidate, int *ppt, int *waves) {
+  if (!e || steps_per_candidate < 1) return PDP3_ERR_BAD_ARG;
+  double saved_ms = e->timed_ms;
+  long long saved_steps = e->timed_steps;
+  int best = 0, st = PDP3_OK;
+  double best_ms = 1e300;
+  for (int t = 0; t < kNumTiles; t++) {
+    e->tile = t;
+    e->timed_ms = 0;
+    e->timed_steps = 0;
+    if ((st = pdp3_ensemble_step(e, steps_per_candidate))) break;
+    if (e->timed_ms < best_ms) {
+      best_ms = e->timed_ms;
+      best = t;
+    }
+  }
+  e->tile = best;
+  e->timed_ms = saved_ms;
+  e->timed_steps = saved_steps;
+  if (st) return st;
+  return pdp3_ensemble_get_tile(e, ppt, waves);
 }
 
 int pdp3_profile_enable(pdp3_ctx *ctx, int enable) {

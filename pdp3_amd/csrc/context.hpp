@@ -152,6 +152,22 @@ void launch_e_update(Ctx &c);
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
 void launch_gauss_residual(Ctx &c, const double *rho, double *out);
 
+// ensembles of independent instances: device tables of every instance's
+// grid (built once; the field pointers of a context never move)
+struct EnsFields;
+EnsFields *ens_fields_build(const std::vector<Ctx *> &ctxs);  // nullptr: out of memory
+void ens_fields_free(EnsFields *e);
+void ens_launch_h(EnsFields *e, hipStream_t st);
+void ens_launch_e(EnsFields *e, hipStream_t st);
+void ens_launch_zero(EnsFields *e, hipStream_t st);
+// the plasma tiles and bunch blocks of every instance, one persistent launch;
+// rebuild after a sort (particle arrays move)
+struct EnsParticles;
+EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs);
+int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs);  // 0 on success
+void ens_particles_free(EnsParticles *e);
+void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st);
+
 // FP64 plain form
 void launch_fused_f64(Ctx &c, Species &sp);
 void launch_step_v_f64(Ctx &c, Species &sp);

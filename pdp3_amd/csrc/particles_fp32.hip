@@ -15,6 +15,9 @@
 // through dy^2/(R+X)); the near-axis cell (1/r, the CYL_VOL annulus weights)
 // is evaluated in FP64; the grid accumulation is FP64 (hardware FP64
 // atomics); fields and the PML stay FP64.
+#include <algorithm>
+#include <vector>
+
 #include "context.hpp"
 #include "sink.hpp"
 
@@ -555,9 +558,12 @@ __global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs,
 
 // bunches (plain layout): the same walk, the box anchored at the smallest
 // living cell of the block
-__global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs, long long n_cap) {
-  __shared__ double box[BunchSink::kSize];
-  const long long base = (long long)blockIdx.x * kBlockParticles;
+// one block of a bunch (plain layout): WAVES chunks of kChunk slots from
+// `base`; the register / LDS sink is anchored at the smallest cell of the
+// block's living particles (a bunch block is a thin disc injected in one step)
+template <int WAVES>
+__device__ __forceinline__ void fused32_bunch_tile(const F32Args &p, const GlobalSink &gs, long long base,
+                                                   long long n_cap, double *box) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long long cbase = base + (long long)wave * kChunk + lane;
   int ai = 0x7fffffff, ak = 0x7fffffff;
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs,
     }
   }
   block_min2(ai, ak);
-  if (ai == 0x7fffffff) return;  // no living particle in the block
+  if (ai == 0x7fffffff) return;  // no living particle in the block (uniform over the block)
   BunchSink sk;
   box_init(box, ai, ak, sk, gs);
   __syncthreads();
@@ -583,6 +589,38 @@ __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs,
   sk.flush();
   __syncthreads();
   box_flush(box, sk);
+}
+
+__global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs, long long n_cap) {
+  __shared__ double box[BunchSink::kSize];
+  fused32_bunch_tile<kWavesPerBlock>(p, gs, (long long)blockIdx.x * kBlockParticles, n_cap, box);
+}
+
+// ensembles: the plasma tiles and bunch blocks of every instance in one
+// persistent launch; blocks walk the item list with a stride of the grid
+struct EnsItem {
+  int arg;   // index into the per-(instance, species) argument tables
+  int kind;  // 0 plasma tile, 1 bunch block
+  long long base, n_cap;
+};
+static_assert(BunchSink::kSize == TileSink::kSize, "the two sinks share one LDS box");
+
+template <int PPT, int WAVES>
+__global__ void __launch_bounds__(256) k_ens_fused32(const F32Args *__restrict__ args,
+                                                     const GlobalSink *__restrict__ sinks,
+                                                     const EnsItem *__restrict__ items, int n_items) {
+  __shared__ double box[TileSink::kSize];
+  __shared__ float fbox[kFSize];
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const EnsItem item = items[it];
+    const F32Args p = args[item.arg];
+    const GlobalSink gs = sinks[item.arg];
+    if (item.kind == 0)
+      fused32_tile<PPT, WAVES>(p, gs, item.base, item.n_cap, box, fbox);
+    else
+      fused32_bunch_tile<WAVES>(p, gs, item.base, item.n_cap, box);
+    __syncthreads();  // the next item clears the boxes
+  }
 }
 
 // plain layout (bunches): one particle per thread, deposits straight to HBM
@@ -803,6 +841,126 @@ void launch_f64_from_f32(Ctx &c, Species &sp) {
                      sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w, sp.f64.alive, sp.n, sp.chunked ? 1 : 0,
                      (const uint32_t *)sp.f32.cell, sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz,
                      sp.f32.w);
+}
+
+struct EnsParticles {
+  int n_args = 0;
+  F32Args *args = nullptr;
+  GlobalSink *sinks = nullptr;
+  EnsItem *items[kNumTiles] = {};
+  int n_items[kNumTiles] = {};
+  int blocks[kNumTiles] = {};
+};
+
+namespace {
+template <int PPT, int WAVES>
+int ens_blocks() {
+  int per_cu = 0, cus = 0, dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ens_fused32<PPT, WAVES>, WAVES * 64, 0);
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+int ens_blocks_of(int tile) {
+  switch (tile) {
+    case 0: return ens_blocks<16, 4>();
+    case 1: return ens_blocks<16, 2>();
+    case 2: return ens_blocks<16, 1>();
+    case 3: return ens_blocks<8, 4>();
+    case 4: return ens_blocks<8, 2>();
+    default: return ens_blocks<4, 4>();
+  }
+}
+}  // namespace
+
+int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs) {
+  std::vector<F32Args> args;
+  std::vector<GlobalSink> sinks;
+  std::vector<EnsItem> items[kNumTiles];
+  for (Ctx *cp : ctxs) {
+    Ctx &c = *cp;
+    for (int j : c.order) {
+      Species &sp = c.species[j];
+      if (sp.n_cap <= 0 || !sp.f32.cell) continue;
+      const int a = (int)args.size();
+      args.push_back(args32(c, sp));
+      sinks.push_back(sink32(c));
+      for (int t = 0; t < kNumTiles; t++) {
+        const int ppt = kTileShapes[t][0], waves = kTileShapes[t][1];
+        const long long tile = sp.chunked ? (long long)waves * ppt / kPPT * kChunk : (long long)waves * kChunk;
+        for (long long b = 0; b < sp.n_cap; b += tile) items[t].push_back(EnsItem{a, sp.chunked ? 0 : 1, b, sp.n_cap});
+      }
+    }
+  }
+  if ((int)args.size() != e->n_args) return -1;  // the species set may not change
+  if (hipMemcpy(e->args, args.data(), args.size() * sizeof(F32Args), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->sinks, sinks.data(), sinks.size() * sizeof(GlobalSink), hipMemcpyHostToDevice) != hipSuccess)
+    return -1;
+  for (int t = 0; t < kNumTiles; t++) {
+    if ((int)items[t].size() != e->n_items[t]) return -1;
+    if (hipMemcpy(e->items[t], items[t].data(), items[t].size() * sizeof(EnsItem), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return -1;
+  }
+  return 0;
+}
+
+EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
+  auto *e = new EnsParticles;
+  int n_args = 0;
+  long long n_items[kNumTiles] = {};
+  for (Ctx *cp : ctxs)
+    for (int j : cp->order) {
+      Species &sp = cp->species[j];
+      if (sp.n_cap <= 0 || !sp.f32.cell) continue;
+      n_args++;
+      for (int t = 0; t < kNumTiles; t++) {
+        const int ppt = kTileShapes[t][0], waves = kTileShapes[t][1];
+        const long long tile = sp.chunked ? (long long)waves * ppt / kPPT * kChunk : (long long)waves * kChunk;
+        n_items[t] += (sp.n_cap + tile - 1) / tile;
+      }
+    }
+  e->n_args = n_args;
+  bool ok = hipMalloc((void **)&e->args, std::max(1, n_args) * sizeof(F32Args)) == hipSuccess &&
+            hipMalloc((void **)&e->sinks, std::max(1, n_args) * sizeof(GlobalSink)) == hipSuccess;
+  for (int t = 0; ok && t < kNumTiles; t++) {
+    e->n_items[t] = (int)n_items[t];
+    ok = hipMalloc((void **)&e->items[t], std::max(1LL, n_items[t]) * sizeof(EnsItem)) == hipSuccess;
+    e->blocks[t] = ens_blocks_of(t);
+  }
+  if (!ok || ens_particles_update(e, ctxs) != 0) {
+    ens_particles_free(e);
+    return nullptr;
+  }
+  return e;
+}
+
+void ens_particles_free(EnsParticles *e) {
+  if (!e) return;
+  if (e->args) (void)hipFree(e->args);
+  if (e->sinks) (void)hipFree(e->sinks);
+  for (auto *p : e->items)
+    if (p) (void)hipFree(p);
+  delete e;
+}
+
+template <int PPT, int WAVES>
+void ens_launch_t(EnsParticles *e, int t, hipStream_t st) {
+  const int blocks = std::min(e->n_items[t], e->blocks[t]);
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL((k_ens_fused32<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, e->args, e->sinks,
+                     e->items[t], e->n_items[t]);
+}
+
+void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st) {
+  switch (tile) {
+    case 0: ens_launch_t<16, 4>(e, 0, st); break;
+    case 1: ens_launch_t<16, 2>(e, 1, st); break;
+    case 2: ens_launch_t<16, 1>(e, 2, st); break;
+    case 3: ens_launch_t<8, 4>(e, 3, st); break;
+    case 4: ens_launch_t<8, 2>(e, 4, st); break;
+    default: ens_launch_t<4, 4>(e, 5, st); break;
+  }
 }
 
 }  // namespace pdp3

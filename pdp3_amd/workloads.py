@@ -172,3 +172,70 @@ def build_simulation(w: Workload, precision: int = 0, sort_interval: int = 0):
     if sort_interval:
         sim.set_sort_interval(sort_interval)
     return sim
+
+
+@dataclass
+class Instance:
+    """One member of the cfg4 ensemble."""
+    cells_r: int
+    cells_z: int
+    ppc: int
+    density: float
+    bunch_ratio: float
+    seed: int
+    dr: float = 5e-4
+    pml_cells: int = 8
+    temperature_ev: float = 1.0
+    bunch_macro: int = 20000
+    sigma_r: float = 1e-3
+    sigma_z: float = 2e-3
+    bunch_velocity: float = 0.99 * LIGHT_SPEED
+
+    @property
+    def particles(self):
+        return self.ppc * self.cells_r * self.cells_z + self.bunch_macro
+
+
+def ensemble_config(n_instances: int = 128, seed: int = DEFAULT_SEED):
+    """cfg4: per-instance draws from the seed: r cells in {64, 128, 256}, z
+    cells in {512, 1024, 2048, 4096}, dr = dz = 5e-4 m, electron density
+    log-uniform in 1e16..1e18 m^-3, 5..40 ppc, 1 eV, bunch / plasma density
+    ratio uniform in 0.01..0.5, Gaussian bunch (sigma_r 1 mm, sigma_z 2 mm) at
+    0.99 c."""
+    rng = np.random.default_rng(seed + 4)
+    out = []
+    for j in range(n_instances):
+        out.append(Instance(cells_r=int(rng.choice([64, 128, 256])), cells_z=int(rng.choice([512, 1024, 2048, 4096])),
+                            ppc=int(rng.integers(5, 41)), density=float(10 ** rng.uniform(16, 18)),
+                            bunch_ratio=float(rng.uniform(0.01, 0.5)), seed=seed + 1009 * j))
+    return out
+
+
+def loaded_gaussian_bunch(rng, inst: Instance):
+    """The Gaussian bunch already inside the grid: its 6 sigma_z window starts
+    at z = dz/2, every particle moves at the bunch velocity along z."""
+    dr = dz = inst.dr
+    b = gaussian_bunch(rng, inst.bunch_macro, inst.sigma_r, inst.sigma_z, inst.bunch_ratio * inst.density,
+                       inst.bunch_velocity, dr, dz)
+    n = inst.bunch_macro
+    z = (np.arange(n) + 0.5) / n * (6.0 * inst.sigma_z) + dz / 2
+    zero = np.zeros(n)
+    return b.r, z, zero, zero, np.full(n, inst.bunch_velocity), b.w
+
+
+def build_instance(inst: Instance, precision: int = 1):
+    """A Simulation of one ensemble member (FP32 form by default)."""
+    from .capi import Simulation
+
+    w = Workload("cfg4", inst.cells_r, inst.cells_z, inst.dr, inst.dr, 1000, inst.pml_cells, 0, seed=inst.seed)
+    sim = Simulation(w.n_grid_r, w.n_grid_z, w.r_size, w.z_size, w.dt, pusher=0, precision=precision)
+    pml = w.pml()
+    if pml:
+        sim.set_pml(*pml)
+    sim.boundary_conditions()
+    idx = sim.add_species(-1, 1.0, inst.ppc * inst.cells_r * inst.cells_z)
+    sim.init_uniform(idx, inst.seed, inst.ppc, inst.density, 0, inst.temperature_ev)
+    rng = np.random.default_rng(inst.seed + 1)
+    sim.add_bunch_loaded(-1, 1.0, *loaded_gaussian_bunch(rng, inst))
+    return sim
+

@@ -1,0 +1,94 @@
+"""Ensembles: independent instances of different shapes advanced together
+(graph replay, one persistent particle launch) give the same result as each
+instance stepped alone, with and without periodic cell sorts, for every tile
+shape the autotuner may pick."""
+import numpy as np
+import pytest
+
+from helpers import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances():
+    from pdp3_amd import workloads as wl
+
+    return [wl.Instance(cells_r=16, cells_z=128, ppc=12, density=1e17, bunch_ratio=0.2, seed=11, bunch_macro=3000),
+            wl.Instance(cells_r=32, cells_z=64, ppc=5, density=3e16, bunch_ratio=0.5, seed=12, bunch_macro=2000),
+            wl.Instance(cells_r=8, cells_z=256, ppc=20, density=1e18, bunch_ratio=0.05, seed=13, bunch_macro=4000)]
+
+
+def _compare(a, b, tol):
+    for name in ("E_r", "E_z", "H_phi", "J_z", "rho_bunch"):
+        e = normwise(a.field(name), b.field(name))
+        assert e <= tol, (name, e)
+    for j in range(2):
+        sa, sb = a.download(j), b.download(j)
+        assert np.array_equal(sa["alive"], sb["alive"])
+        for k in ("r", "z", "vr", "vz"):
+            assert normwise(sa[k], sb[k]) <= tol, (j, k)
+
+
+@pytest.mark.parametrize("sort_interval", [0, 5])
+def test_ensemble_matches_single_instances(sort_interval):
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    insts = _instances()
+    print("ensemble of", len(insts), "sort_interval", sort_interval, flush=True)
+    alone = [wl.build_instance(i) for i in insts]
+    members = [wl.build_instance(i) for i in insts]
+    for s in alone:
+        s.set_sort_interval(sort_interval)
+        s.step(12)
+    ens = capi.Ensemble(members)
+    ens.set_sort_interval(sort_interval)
+    ens.step(12)
+    ms, n = ens.timing()
+    assert n == 12 and ms > 0
+    for a, m in zip(alone, members):
+        assert abs(a.time - m.time) == 0.0
+        _compare(m, a, 1e-10)
+    ens.close()
+
+
+def test_ensemble_tiles_and_autotune():
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    insts = _instances()
+    ref = None
+    for ppt, waves in [(16, 4), (16, 1), (8, 2), (4, 4)]:
+        print("ensemble tile", ppt, waves, flush=True)
+        members = [wl.build_instance(i) for i in insts]
+        ens = capi.Ensemble(members)
+        ens.set_tile(ppt, waves)
+        assert ens.tile() == (ppt, waves)
+        ens.step(8)
+        ens.close()
+        if ref is None:
+            ref = members
+            continue
+        for a, b in zip(members, ref):
+            _compare(a, b, 1e-10)
+    members = [wl.build_instance(i) for i in insts]
+    ens = capi.Ensemble(members)
+    best = ens.autotune(1)
+    assert best in [(16, 4), (16, 2), (16, 1), (8, 4), (8, 2), (4, 4)]
+    ens.step(2)
+    assert members[0].time > 0
+    ens.close()
+
+
+def test_ensemble_rejects_fp64_and_injected_bunches():
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    inst = _instances()[0]
+    s64 = wl.build_instance(inst, precision=capi.PREC_FP64)
+    with pytest.raises(capi.Pdp3Error):
+        capi.Ensemble([s64])
+    w = wl.config("cfg1")
+    s = wl.build_simulation(w, precision=capi.PREC_FP32)   # its bunch is injected over time
+    with pytest.raises(capi.Pdp3Error):
+        capi.Ensemble([s])
