@@ -217,6 +217,27 @@ int pdp3_set_tile(pdp3_ctx *ctx, int ppt, int waves);
 int pdp3_get_tile(const pdp3_ctx *ctx, int *ppt, int *waves);
 int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves);
 
+/* ---- adjoint (no reference counterpart) ----
+ * Runs nsteps loop iterations from the current state and returns the
+ * objective
+ *   phi = dt dz sum over the last window_steps iterations of
+ *         sum over axis nodes z_lo <= (k + 1/2) dz <= z_hi of E_z(0, k)^2
+ * (E after each iteration) and its gradient with respect to the weight of
+ * every particle (real particles per macro-particle; charge_array and
+ * mass_array of src/particles.cpp:328-329 scale with it): dphi_dw[j] receives
+ * species j's n values in its logical particle order (a NULL entry skips the
+ * species).  Hand-written adjoint kernels of the FDTD / PML update and of the
+ * push + deposit (the FP64 plain form's arithmetic, linearised around the
+ * context's own forward trajectory: FP32 or FP64 form), with checkpoints of
+ * the full state every checkpoint_every steps and recomputation between
+ * them.  The context ends in the forward end state.  Requires
+ * sort_interval 0 (a fixed particle order). */
+int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo,
+                     double z_hi, double *phi_out, double *const *dphi_dw);
+
+/* the same objective without the gradient (forward only) */
+int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi, double *phi_out);
+
 /* ---- ensembles (no reference counterpart) ----
  * Independent instances (contexts set up with the calls above: any grid,
  * PML, plasma, loaded bunches; FP32 form, one common delta_t) advanced

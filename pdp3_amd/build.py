@@ -53,6 +53,7 @@ UNITS = [
     ("init.hip", ["-ffp-contract=off"]),
     ("particles_fp32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sort.hip", []),
+    ("adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 

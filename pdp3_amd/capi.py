@@ -68,6 +68,8 @@ SIGNATURES = {
     "pdp3_set_sort_interval": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
     "pdp3_bunch_add_loaded": (_I, [_P, _I, _F, _I, _D, _D, _D, _D, _D, _D, C.POINTER(_I)]),
+    "pdp3_adjoint_run": (_I, [_P, _I, _I, _I, _F, _F, _D, C.POINTER(_D)]),
+    "pdp3_objective_run": (_I, [_P, _I, _I, _F, _F, _D]),
     "pdp3_ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
     "pdp3_ensemble_destroy": (_I, [_P]),
     "pdp3_ensemble_last_error": (C.c_char_p, [_P]),
@@ -202,6 +204,7 @@ class Simulation:
     def add_species(self, charge_e, mass_e, number):
         idx = _I()
         self._chk(self.L.pdp3_species_add(self.h, int(charge_e), float(mass_e), int(number), C.byref(idx)))
+        self._nsp = max(getattr(self, '_nsp', 0), idx.value + 1)
         return idx.value
 
     def add_plasma(self, charge_e, mass_e, r, z, vr, vphi, vz, w, alive=None):
@@ -216,6 +219,7 @@ class Simulation:
         idx = _I()
         self._chk(self.L.pdp3_bunch_add_loaded(self.h, int(charge_e), float(mass_e), len(arrs[0]),
                                                *(_dp(a) for a in arrs), C.byref(idx)))
+        self._nsp = max(getattr(self, '_nsp', 0), idx.value + 1)
         return idx.value
 
     def add_bunch(self, charge_e, mass_e, r, w, inject_u, duration, hole_duration, velocity, bunch_number=0):
@@ -224,11 +228,15 @@ class Simulation:
         self._chk(self.L.pdp3_bunch_add(self.h, int(charge_e), float(mass_e), len(r), _dp(r), _dp(w), _dp(u),
                                         float(duration), float(hole_duration), float(velocity), int(bunch_number),
                                         C.byref(idx)))
+        self._nsp = max(getattr(self, '_nsp', 0), idx.value + 1)
         return idx.value
 
     def init_uniform(self, idx, seed, ppc, density, ramp_cells, temperature_ev):
         self._chk(self.L.pdp3_species_init_uniform(self.h, idx, C.c_uint64(seed), int(ppc), float(density),
                                                    int(ramp_cells), float(temperature_ev)))
+
+    def n_species(self):
+        return getattr(self, "_nsp", 0)
 
     def count(self, idx):
         n = _I()
@@ -270,6 +278,23 @@ class Simulation:
         rho = np.empty((self.nr, self.nz))
         self._chk(self.L.pdp3_gauss_residual(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
         return R, rho
+
+    def adjoint_run(self, nsteps, checkpoint_every, window_steps, z_lo, z_hi):
+        """(phi, [dphi/dw per species]) of the axis E_z^2 objective over the
+        last window_steps of nsteps iterations (the simulation advances)."""
+        counts = [self.count(j) for j in range(self.n_species())]
+        outs = [np.zeros(max(n, 1)) for n in counts]
+        arr = (_D * len(outs))(*[_dp(o) for o in outs])
+        phi = C.c_double()
+        self._chk(self.L.pdp3_adjoint_run(self.h, int(nsteps), int(checkpoint_every), int(window_steps),
+                                          float(z_lo), float(z_hi), C.byref(phi), arr))
+        return phi.value, [o[:n] for o, n in zip(outs, counts)]
+
+    def objective_run(self, nsteps, window_steps, z_lo, z_hi):
+        phi = C.c_double()
+        self._chk(self.L.pdp3_objective_run(self.h, int(nsteps), int(window_steps), float(z_lo), float(z_hi),
+                                            C.byref(phi)))
+        return phi.value
 
     def set_tile(self, ppt, waves):
         self._chk(self.L.pdp3_set_tile(self.h, int(ppt), int(waves)))

@@ -5,11 +5,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "context.hpp"
@@ -824,23 +826,38 @@ int pdp3_bunch_inject(pdp3_ctx *ctx, int index) {
   return inject_bunch(ctx, ctx->c.species[index]);
 }
 
+}  // extern "C"
+
+namespace {
+// one loop iteration (src/loadInitParam.cpp:471-587); after_h (may be null)
+// runs right after the H update, when E^n and H at the half time are final
+int step_one(pdp3_ctx *ctx, const std::function<int()> *after_h) {
+  Ctx &c = ctx->c;
+  int st;
+  if (c.sort_interval > 0 && c.steps_done > 0 && c.steps_done % c.sort_interval == 0) {
+    if ((st = pdp3_sort_particles(ctx))) return st;
+  }
+  if ((st = init_beam(ctx))) return st;
+  if ((st = pdp3_hfield_calc(ctx))) return st;
+  if (after_h && (st = (*after_h)())) return st;
+  if ((st = pdp3_current_reset(ctx))) return st;
+  if ((st = zero_grid(ctx, kRhoBunch))) return st;
+  for (int j : c.order)
+    if ((st = particles_fused(ctx, c.species[j]))) return st;
+  if ((st = pdp3_efield_calc(ctx))) return st;
+  c.current_time = c.current_time + c.g.dt;  // src/loadInitParam.cpp:586
+  c.steps_done++;
+  return PDP3_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int pdp3_step(pdp3_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return PDP3_ERR_BAD_ARG;
-  Ctx &c = ctx->c;
   for (int n = 0; n < nsteps; n++) {
-    int st;
-    if (c.sort_interval > 0 && c.steps_done > 0 && c.steps_done % c.sort_interval == 0) {
-      if ((st = pdp3_sort_particles(ctx))) return st;
-    }
-    if ((st = init_beam(ctx))) return st;
-    if ((st = pdp3_hfield_calc(ctx))) return st;
-    if ((st = pdp3_current_reset(ctx))) return st;
-    if ((st = zero_grid(ctx, kRhoBunch))) return st;
-    for (int j : c.order)
-      if ((st = particles_fused(ctx, c.species[j]))) return st;
-    if ((st = pdp3_efield_calc(ctx))) return st;
-    c.current_time = c.current_time + c.g.dt;  // src/loadInitParam.cpp:586
-    c.steps_done++;
+    int st = step_one(ctx, nullptr);
+    if (st) return st;
   }
   return check_device_errors(ctx);
 }
@@ -916,6 +933,275 @@ int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves) 
   c.timers[PDP3_KERNEL_PARTICLES].launches = saved.launches;
   if (st) return st;
   return pdp3_get_tile(ctx, ppt, waves);
+}
+
+// ----------------------------------------------------------------- adjoint
+
+}  // extern "C"
+
+namespace {
+
+// a device buffer owned by the adjoint driver
+struct DBuf {
+  void *p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+struct SpeciesCkpt {
+  std::vector<DBuf> bufs;  // native state arrays
+};
+
+// the state a loop iteration reads: fields E, H (and the rest of the grid for
+// the final restore) and every species' native particle arrays, plus the host
+// bookkeeping of time and bunches
+struct Checkpoint {
+  DBuf grid;  // components kER .. kRhoBunch
+  std::vector<SpeciesCkpt> sp;
+  double time = 0;
+  long long steps = 0;
+  int next_bunch = 0;
+  std::vector<int> order;
+  std::vector<char> created;
+};
+
+constexpr int kCkComps = kRhoBunch + 1;
+
+std::vector<std::pair<void *, size_t>> native_arrays(Ctx &c, Species &sp) {
+  std::vector<std::pair<void *, size_t>> out;
+  const size_t n = (size_t)sp.n_cap;
+  if (is_f32(c)) {
+    out.push_back({sp.f32.cell, n * sizeof(int32_t)});
+    for (float *a : {sp.f32.fr, sp.f32.fz, sp.f32.vr, sp.f32.vphi, sp.f32.vz}) out.push_back({a, n * sizeof(float)});
+  } else {
+    for (double *a : {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz}) out.push_back({a, n * sizeof(double)});
+    out.push_back({sp.f64.alive, n});
+  }
+  return out;
+}
+
+int ckpt_save(pdp3_ctx *ctx, Checkpoint &ck) {
+  Ctx &c = ctx->c;
+  const size_t gb = kCkComps * c.cells() * sizeof(double);
+  if (!ck.grid.p) HIP_TRY(ctx, hipMalloc(&ck.grid.p, gb));
+  HIP_TRY(ctx, hipMemcpyAsync(ck.grid.p, c.a[0], gb, hipMemcpyDeviceToDevice, c.stream));
+  if (ck.sp.empty()) ck.sp.resize(c.species.size());
+  for (size_t j = 0; j < c.species.size(); j++) {
+    auto arrs = native_arrays(c, c.species[j]);
+    auto &b = ck.sp[j].bufs;
+    if (b.empty()) {
+      b = std::vector<DBuf>(arrs.size());
+      for (size_t a = 0; a < arrs.size(); a++) HIP_TRY(ctx, hipMalloc(&b[a].p, std::max<size_t>(arrs[a].second, 1)));
+    }
+    for (size_t a = 0; a < arrs.size(); a++)
+      if (arrs[a].second)
+        HIP_TRY(ctx, hipMemcpyAsync(b[a].p, arrs[a].first, arrs[a].second, hipMemcpyDeviceToDevice, c.stream));
+  }
+  ck.time = c.current_time;
+  ck.steps = c.steps_done;
+  ck.next_bunch = c.next_bunch;
+  ck.order = c.order;
+  ck.created.clear();
+  for (auto &sp : c.species) ck.created.push_back(sp.created ? 1 : 0);
+  return PDP3_OK;
+}
+
+int ckpt_restore(pdp3_ctx *ctx, const Checkpoint &ck) {
+  Ctx &c = ctx->c;
+  const size_t gb = kCkComps * c.cells() * sizeof(double);
+  HIP_TRY(ctx, hipMemcpyAsync(c.a[0], ck.grid.p, gb, hipMemcpyDeviceToDevice, c.stream));
+  for (size_t j = 0; j < c.species.size(); j++) {
+    auto arrs = native_arrays(c, c.species[j]);
+    for (size_t a = 0; a < arrs.size(); a++)
+      if (arrs[a].second)
+        HIP_TRY(ctx, hipMemcpyAsync(arrs[a].first, ck.sp[j].bufs[a].p, arrs[a].second, hipMemcpyDeviceToDevice,
+                                    c.stream));
+  }
+  c.current_time = ck.time;
+  c.steps_done = ck.steps;
+  c.next_bunch = ck.next_bunch;
+  c.order = ck.order;
+  for (size_t j = 0; j < c.species.size(); j++) c.species[j].created = ck.created[j] != 0;
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+// FP64 particle state of one step, in the index space of the species (slots
+// of the FP64 form, logical order of the FP32 form's staging)
+struct SpeciesSnap {
+  DBuf r, z, vr, vphi, vz, alive;
+};
+
+long long adj_count(const Ctx &c, const Species &sp) { return is_f32(c) ? sp.n : sp.n_cap; }
+
+}  // namespace
+
+extern "C" {
+
+int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo, double z_hi,
+                     double *phi_out, double *const *dphi_dw) {
+  if (!ctx || nsteps < 1 || checkpoint_every < 1 || window_steps < 0 || window_steps > nsteps || !(z_hi >= z_lo))
+    return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (c.sort_interval != 0) return fail(ctx, PDP3_ERR_STATE, "the adjoint needs a fixed particle order (sort_interval 0)");
+  int st;
+  if ((st = check_device_errors(ctx))) return st;
+  const size_t cells = c.cells();
+  const int nsp = (int)c.species.size();
+  // axis window of the objective: E_z(0, k) sits at z = (k + 1/2) dz
+  int k0 = (int)std::ceil(z_lo / c.g.dz - 0.5), k1 = (int)std::floor(z_hi / c.g.dz - 0.5);
+  k0 = std::max(k0, 0);
+  k1 = std::min(k1, c.g.nz - 2);
+  const double scale = c.g.dt * c.g.dz;
+
+  // forward with checkpoints
+  const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+  std::vector<Checkpoint> ck(nck);
+  Checkpoint final_state;
+  for (int n = 0; n < nsteps; n++) {
+    if (n % checkpoint_every == 0 && (st = ckpt_save(ctx, ck[n / checkpoint_every]))) return st;
+    if ((st = step_one(ctx, nullptr))) return st;
+  }
+  if ((st = check_device_errors(ctx)) || (st = ckpt_save(ctx, final_state))) return st;
+
+  // adjoint state
+  DBuf lam_grid, phi_dev, ez_rows;
+  HIP_TRY(ctx, hipMalloc(&lam_grid.p, 12 * cells * sizeof(double)));
+  HIP_TRY(ctx, hipMemsetAsync(lam_grid.p, 0, 12 * cells * sizeof(double), c.stream));
+  double *lam[12];
+  for (int j = 0; j < 12; j++) lam[j] = lam_grid.as<double>() + (size_t)j * cells;
+  HIP_TRY(ctx, hipMalloc(&phi_dev.p, sizeof(double)));
+  HIP_TRY(ctx, hipMemsetAsync(phi_dev.p, 0, sizeof(double), c.stream));
+  const int K = checkpoint_every;
+  HIP_TRY(ctx, hipMalloc(&ez_rows.p, (size_t)K * c.g.nz * sizeof(double)));
+  std::vector<DBuf> lstate((size_t)nsp * 6);  // r, z, vr, vphi, vz, w per species
+  for (int j = 0; j < nsp; j++) {
+    const size_t n = (size_t)std::max<long long>(adj_count(c, c.species[j]), 1);
+    for (int a = 0; a < 6; a++) {
+      HIP_TRY(ctx, hipMalloc(&lstate[j * 6 + a].p, n * sizeof(double)));
+      HIP_TRY(ctx, hipMemsetAsync(lstate[j * 6 + a].p, 0, n * sizeof(double), c.stream));
+    }
+  }
+  // per-step snapshots of one segment: E^n, Hh^n and the FP64 particle states
+  DBuf fsnap;
+  HIP_TRY(ctx, hipMalloc(&fsnap.p, (size_t)K * 6 * cells * sizeof(double)));
+  std::vector<SpeciesSnap> psnap((size_t)K * nsp);
+  for (int t = 0; t < K; t++)
+    for (int j = 0; j < nsp; j++) {
+      const size_t n = (size_t)std::max<long long>(adj_count(c, c.species[j]), 1);
+      SpeciesSnap &s = psnap[(size_t)t * nsp + j];
+      for (DBuf *b : {&s.r, &s.z, &s.vr, &s.vphi, &s.vz}) HIP_TRY(ctx, hipMalloc(&b->p, n * sizeof(double)));
+      HIP_TRY(ctx, hipMalloc(&s.alive.p, n));
+    }
+
+  for (int seg = nck - 1; seg >= 0; seg--) {
+    const int n0 = seg * K, n1 = std::min(nsteps, n0 + K);
+    if ((st = ckpt_restore(ctx, ck[seg]))) return st;
+    // recompute the segment, keeping what each step's adjoint reads
+    for (int n = n0; n < n1; n++) {
+      const int t = n - n0;
+      std::function<int()> snap = [&]() -> int {
+        double *fs = fsnap.as<double>() + (size_t)t * 6 * cells;
+        const int comps[6] = {kER, kEPhi, kEZ, kHHR, kHHPhi, kHHZ};
+        for (int a = 0; a < 6; a++)
+          HIP_TRY(ctx, hipMemcpyAsync(fs + (size_t)a * cells, c.a[comps[a]], cells * sizeof(double),
+                                      hipMemcpyDeviceToDevice, c.stream));
+        for (int j = 0; j < nsp; j++) {
+          Species &sp = c.species[j];
+          SpeciesSnap &s = psnap[(size_t)t * nsp + j];
+          const long long n = adj_count(c, sp);
+          if (n <= 0) continue;
+          if (is_f32(c)) {
+            launch_f64_from_f32(c, sp);  // logical-order FP64 staging
+            CHECK_LAUNCH(ctx);
+          }
+          const double *src[5] = {sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz};
+          DBuf *dst[5] = {&s.r, &s.z, &s.vr, &s.vphi, &s.vz};
+          for (int a = 0; a < 5; a++)
+            HIP_TRY(ctx, hipMemcpyAsync(dst[a]->p, src[a], (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                        c.stream));
+          HIP_TRY(ctx, hipMemcpyAsync(s.alive.p, sp.f64.alive, (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+        }
+        return PDP3_OK;
+      };
+      if ((st = step_one(ctx, &snap))) return st;
+      HIP_TRY(ctx, hipMemcpyAsync(ez_rows.as<double>() + (size_t)t * c.g.nz, c.a[kEZ], c.g.nz * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c.stream));
+    }
+    if ((st = check_device_errors(ctx))) return st;
+    // backward through the segment
+    for (int n = n1 - 1; n >= n0; n--) {
+      const int t = n - n0;
+      if (n >= nsteps - window_steps && k1 >= k0)
+        launch_adj_seed(c.g, ez_rows.as<double>() + (size_t)t * c.g.nz, lam[2], k0, k1, scale, phi_dev.as<double>(),
+                        c.stream);
+      launch_adj_fields_pre(c.g, lam, c.koef_e, c.koef_h, c.stream);
+      const double *fs = fsnap.as<double>() + (size_t)t * 6 * cells;
+      const double *f6[6] = {fs, fs + cells, fs + 2 * cells, fs + 3 * cells, fs + 4 * cells, fs + 5 * cells};
+      for (int j = 0; j < nsp; j++) {
+        Species &sp = c.species[j];
+        SpeciesSnap &s = psnap[(size_t)t * nsp + j];
+        double *ls[5];
+        for (int a = 0; a < 5; a++) ls[a] = lstate[j * 6 + a].as<double>();
+        launch_adj_particles(c.g, f6, lam, s.r.as<double>(), s.z.as<double>(), s.vr.as<double>(),
+                             s.vphi.as<double>(), s.vz.as<double>(), sp.f64.w, s.alive.as<uint8_t>(), ls,
+                             lstate[j * 6 + 5].as<double>(), sp.charge, sp.mass, adj_count(c, sp),
+                             sp.kind == PDP3_SPECIES_BUNCH, c.pusher, c.stream);
+      }
+      launch_adj_fields_post(c.g, lam, c.koef_e, c.koef_h, c.stream);
+      CHECK_LAUNCH(ctx);
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  // results
+  double phi = 0;
+  HIP_TRY(ctx, hipMemcpy(&phi, phi_dev.p, sizeof(double), hipMemcpyDeviceToHost));
+  if (phi_out) *phi_out = phi;
+  if (dphi_dw) {
+    for (int j = 0; j < nsp; j++) {
+      if (!dphi_dw[j]) continue;
+      Species &sp = c.species[j];
+      const long long n = adj_count(c, sp);
+      std::vector<double> h((size_t)std::max<long long>(n, 1));
+      HIP_TRY(ctx, hipMemcpy(h.data(), lstate[j * 6 + 5].p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+      if (!is_f32(c) && sp.chunked) {
+        for (long long p = 0; p < n; p++) {
+          const long long l = logical_of(p);
+          if (l < sp.n) dphi_dw[j][l] = h[(size_t)p];
+        }
+      } else {
+        std::memcpy(dphi_dw[j], h.data(), (size_t)sp.n * sizeof(double));
+      }
+    }
+  }
+  return ckpt_restore(ctx, final_state);
+}
+
+int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi, double *phi_out) {
+  if (!ctx || nsteps < 0 || window_steps < 0 || window_steps > nsteps || !(z_hi >= z_lo) || !phi_out)
+    return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  int k0 = (int)std::ceil(z_lo / c.g.dz - 0.5), k1 = (int)std::floor(z_hi / c.g.dz - 0.5);
+  k0 = std::max(k0, 0);
+  k1 = std::min(k1, c.g.nz - 2);
+  DBuf phi_dev, scratch;
+  HIP_TRY(ctx, hipMalloc(&phi_dev.p, sizeof(double)));
+  HIP_TRY(ctx, hipMalloc(&scratch.p, c.g.nz * sizeof(double)));
+  HIP_TRY(ctx, hipMemsetAsync(phi_dev.p, 0, sizeof(double), c.stream));
+  int st;
+  for (int n = 0; n < nsteps; n++) {
+    if ((st = step_one(ctx, nullptr))) return st;
+    if (n >= nsteps - window_steps && k1 >= k0)
+      launch_adj_seed(c.g, c.a[kEZ], scratch.as<double>(), k0, k1, c.g.dt * c.g.dz, phi_dev.as<double>(), c.stream);
+  }
+  if ((st = check_device_errors(ctx))) return st;
+  HIP_TRY(ctx, hipMemcpy(phi_out, phi_dev.p, sizeof(double), hipMemcpyDeviceToHost));
+  return PDP3_OK;
 }
 
 // ---------------------------------------------------------------- ensembles

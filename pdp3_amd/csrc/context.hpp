@@ -168,6 +168,18 @@ int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs);  // 0
 void ens_particles_free(EnsParticles *e);
 void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st);
 
+// adjoint of one loop iteration (FP64); lam = adjoints of E_r, E_phi, E_z,
+// H_r, H_phi, H_z, Hh_r, Hh_phi, Hh_z, J_r, J_phi, J_z (12 grids)
+void launch_adj_fields_pre(const GridDesc &g, double *const *lam, const double *ke, const double *kh, hipStream_t st);
+void launch_adj_fields_post(const GridDesc &g, double *const *lam, const double *ke, const double *kh,
+                            hipStream_t st);
+void launch_adj_particles(const GridDesc &g, const double *const *fields6, double *const *lam, const double *r,
+                          const double *z, const double *vr, const double *vphi, const double *vz, const double *w,
+                          const uint8_t *alive, double *const *lstate, double *lw, double charge, double mass,
+                          long long n, int is_bunch, int pusher, hipStream_t st);
+void launch_adj_seed(const GridDesc &g, const double *ez, double *lez, int k0, int k1, double scale, double *phi,
+                     hipStream_t st);
+
 // FP64 plain form
 void launch_fused_f64(Ctx &c, Species &sp);
 void launch_step_v_f64(Ctx &c, Species &sp);
