@@ -63,6 +63,7 @@ def lib():
         L.or_sim_field.restype = _D
         L.or_sim_field.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_add_plasma.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int] + [_D] * 6 + [_U8]
+        L.or_sim_add_bunch_loaded.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _D, _D, _D, _D, _D, _D]
         L.or_sim_add_bunch.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _D, _D, _D,
                                        C.c_double, C.c_double, C.c_double, C.c_int]
         L.or_sim_num_species.argtypes = [C.c_void_p]
@@ -155,6 +156,15 @@ class Sim:
         r, w, u = map(_f64, (r, w, inject_u))
         idx = lib().or_sim_add_bunch(self.h, int(charge_e), float(mass_e), len(r), _dp(r), _dp(w), _dp(u),
                                      duration, hole_duration, velocity, int(bunch_number))
+        if idx < 0:
+            raise RuntimeError("too many species")
+        self.species.append(idx)
+        return idx
+
+    def add_bunch_loaded(self, charge_e, mass_e, r, z, vr, vphi, vz, w):
+        arrs = [_f64(a) for a in (r, z, vr, vphi, vz, w)]
+        idx = lib().or_sim_add_bunch_loaded(self.h, int(charge_e), float(mass_e), len(arrs[0]),
+                                            *(_dp(a) for a in arrs))
         if idx < 0:
             raise RuntimeError("too many species")
         self.species.append(idx)

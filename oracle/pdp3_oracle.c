@@ -262,6 +262,35 @@ int or_sim_add_bunch(or_sim *s, int charge_e, double mass_e, int n,
   return s->nsp++;
 }
 
+/* a bunch already inside the grid: created and alive at once, nothing left
+ * to inject (duration 0); absorbed at the walls and charge-weighted like any
+ * bunch.  No reference counterpart (its bunches enter by injection only):
+ * the adjoint checks differentiate a bunch given as a state. */
+int or_sim_add_bunch_loaded(or_sim *s, int charge_e, double mass_e, int n,
+                            const double *r, const double *z, const double *vr,
+                            const double *vphi, const double *vz, const double *w) {
+  species_t *p = new_species(s, n, charge_e, mass_e);
+  if (!p) return OR_ERR_BAD_ARG;
+  p->is_bunch = 1;
+  p->r = dup_or_fill(r, n, 0);
+  p->z = dup_or_fill(z, n, 0);
+  p->vr = dup_or_fill(vr, n, 0);
+  p->vphi = dup_or_fill(vphi, n, 0);
+  p->vz = dup_or_fill(vz, n, 0);
+  p->q = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  p->m = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int j = 0; j < n; j++) {
+    p->q[j] = p->charge * w[j];
+    p->m[j] = p->mass * w[j];
+    p->alive[j] = 1;
+  }
+  p->inject_u = dup_or_fill(NULL, n, 0);
+  p->duration = 0;
+  p->created = 1;
+  s->order[s->norder++] = s->nsp;
+  return s->nsp++;
+}
+
 int or_sim_num_species(const or_sim *s) { return s->nsp; }
 int or_sim_species_count(const or_sim *s, int idx) {
   return (idx >= 0 && idx < s->nsp) ? s->sp[idx].n : -1;
@@ -1197,6 +1226,7 @@ void or_sim_bunch_inject(or_sim *s, int idx) {
   double bunch_time_end = bunch_time_begin + p->duration;
   double dl = p->velocity * s->dt;
   int steps_amount = (int)ceil(p->duration / s->dt);
+  if (steps_amount <= 0) return; /* a loaded bunch: nothing to inject */
   unsigned int particles_in_step = (unsigned int)p->n / (unsigned int)steps_amount;
   int start_number = (int)((s->current_time - bunch_time_begin) / s->dt * particles_in_step);
   double half_z_cell_size = s->dz / 2.0;

@@ -74,6 +74,9 @@ int or_sim_add_plasma(or_sim *s, int charge_e, double mass_e, int n,
  * used for particle i's injection z (src/bunch.cpp:106-117).  The bunch is
  * created in the species list at the first step with
  * t >= bunch_number * (duration + hole_duration) (src/loadInitParam.cpp:219-243). */
+int or_sim_add_bunch_loaded(or_sim *s, int charge_e, double mass_e, int n,
+                            const double *r, const double *z, const double *vr,
+                            const double *vphi, const double *vz, const double *w);
 int or_sim_add_bunch(or_sim *s, int charge_e, double mass_e, int n,
                      const double *r, const double *w, const double *inject_u,
                      double duration, double hole_duration, double velocity,

@@ -458,15 +458,33 @@ __device__ __forceinline__ void walls(const F32Args &p, int &i, float &xa, int &
   }
 }
 
-// the whole step of one particle in slot n
+// the state of one particle as loaded from HBM
+struct PState {
+  uint32_t c;
+  float a, b, u0, u1, u2, w;
+};
+
+__device__ __forceinline__ PState load_state(const F32Args &p, long long n) {
+  PState s;
+  s.c = p.cell[n];
+  s.a = p.fa[n];
+  s.b = p.fb[n];
+  s.u0 = p.ur[n];
+  s.u1 = p.uphi[n];
+  s.u2 = p.uz[n];
+  s.w = p.w[n];
+  return s;
+}
+
+// the whole step of one particle (state st, slot n)
 template <class S, class FS>
-__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, long long n) {
-  uint32_t c = p.cell[n];
+__device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &fs, const PState &ps, long long n) {
+  const uint32_t c = ps.c;
   if (c & kDead) return;
   int i = cell_i(c), k = cell_k(c);
-  float a = p.fa[n], b = p.fb[n];
-  float u[3] = {p.ur[n], p.uphi[n], p.uz[n]};
-  const float w = p.w[n];
+  float a = ps.a, b = ps.b;
+  float u[3] = {ps.u0, ps.u1, ps.u2};
+  const float w = ps.w;
   sk.begin_cell(i, k);
   // gathers (E at the full step, H at the half-time copy)
   RadW rl = rad_lo(i, a), ro = rad_own(i, a);
@@ -513,6 +531,11 @@ __device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const F
 
 // chunked plasma species: 4 wavefronts per block, each walking one chunk of
 // kChunk slots kPPT particles per lane, deposits through the tile sink
+template <class S, class FS>
+__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, long long n) {
+  step_state32(p, sk, fs, load_state(p, n), n);
+}
+
 // one tile of a cell-sorted (chunked) species: CB = WAVES * PPT / kPPT chunks
 // starting at slot `base`.  Wave w takes chunk w / (kPPT / PPT) and, in it,
 // the slots of iterations j0 .. j0+PPT-1, j0 = (w % (kPPT / PPT)) * PPT: its
@@ -539,17 +562,27 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
   constexpr int kSplit = kPPT / PPT;
   const int chunk = wave / kSplit, j0 = (wave % kSplit) * PPT;
   const long long cbase = base + (long long)chunk * kChunk + (long long)j0 * 64 + lane;
+  // the loads of particle j+1 are issued before particle j is processed, so
+  // their HBM latency overlaps its arithmetic (the compiler may not hoist them
+  // itself: the stores of particle j could alias them)
+  PState cur = cbase < n_cap ? load_state(p, cbase) : PState{kDead, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int j = 0; j < PPT; j++) {
-    long long n = cbase + (long long)j * 64;
-    if (n < n_cap) step_particle32(p, sk, fs, n);
+    const long long n = cbase + (long long)j * 64;
+    const long long nn = n + 64;
+    PState nxt{kDead, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (j + 1 < PPT && nn < n_cap) nxt = load_state(p, nn);
+    if (n < n_cap) step_state32(p, sk, fs, cur, n);
+    cur = nxt;
   }
   sk.flush();
   __syncthreads();
   box_flush(box, sk);
 }
 
+// 4 waves per SIMD (128 VGPRs): the occupancy that hides the loads best
 template <int PPT, int WAVES>
-__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
   __shared__ double box[TileSink::kSize];
   __shared__ float fbox[kFSize];
   constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
@@ -606,7 +639,7 @@ struct EnsItem {
 static_assert(BunchSink::kSize == TileSink::kSize, "the two sinks share one LDS box");
 
 template <int PPT, int WAVES>
-__global__ void __launch_bounds__(256) k_ens_fused32(const F32Args *__restrict__ args,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_ens_fused32(const F32Args *__restrict__ args,
                                                      const GlobalSink *__restrict__ sinks,
                                                      const EnsItem *__restrict__ items, int n_items) {
   __shared__ double box[TileSink::kSize];
