@@ -112,6 +112,115 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
+def traffic_from_profile(name):
+    """HBM bytes per launch of the dominant kernel as measured by rocprofv3
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a
+    device-to-device copy of known size), read from the committed summary
+    profiles/<name>.md (line `traffic_bytes_per_launch: N`); None when absent."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        return None
+    return None
+
+
+def bench_ensemble(a):
+    """cfg4: 128 independent instances (per-instance grid, density, ppc, bunch
+    ratio drawn from the seed), FP32 form, one graph replay per step with one
+    persistent particle launch; the tile shape picked by the autotuner."""
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    insts = wl.ensemble_config(a.instances)
+    n_part = sum(i.particles for i in insts)
+    cells = sum(i.cells_r * i.cells_z for i in insts)
+    print(f"# ensemble of {len(insts)} instances, {n_part} particles, {cells} cells", file=sys.stderr, flush=True)
+    sims = [wl.build_instance(i) for i in insts]
+    ens = capi.Ensemble(sims)
+    ens.set_sort_interval(a.sort_interval)
+    tile = ens.autotune(1)
+    ens.step(a.warmup)
+    ens.timing()
+    t0 = time.perf_counter()
+    ens.step(a.steps)
+    t1 = time.perf_counter()
+    dev_ms, n = ens.timing()
+    value = n_part * a.steps / (t1 - t0)
+    out = {
+        "metric": "particle-steps/s (ensemble, FP32 form)", "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * (t1 - t0) / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"cfg4: {len(insts)} r-z instances, r cells in 64/128/256 x z cells in "
+                               f"512..4096, 5..40 ppc, densities 1e16..1e18, Gaussian bunches",
+                   "particles": n_part, "cells": cells, "tile_ppt_waves": list(tile),
+                   "sort_interval": a.sort_interval},
+        "device_ms_per_step": dev_ms / max(n, 1),
+        "roofline": {"bound": "hbm", "achieved": (n_part * BYTES_PER_PARTICLE["fp32"] + cells * 304.0)
+                     / (dev_ms / max(n, 1) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "note": "whole step (H, J reset, particles, E) against 52 B per particle + 304 B per cell"},
+    }
+    out["roofline"]["frac"] = out["roofline"]["achieved"] / HBM_PEAK_GBS
+    ens.close()
+    for s_ in sims:
+        s_.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(a):
+    """cfg5: gradient of the axis E_z^2 objective with respect to 16 bump
+    amplitudes and the ramp slope (FP32 forward, checkpoints every 50 steps,
+    hand-written adjoint), timed, and checked against central differences of
+    the FP64 plain form (relative step 1e-4) on the components in --fd."""
+    import torch
+
+    from pdp3_amd import adjoint as adj
+    from pdp3_amd import capi
+
+    spec = adj.cfg5_spec()
+    prob = adj.BumpRampProblem(spec)
+    th = prob.theta0()
+    t0 = time.perf_counter()
+    phi_fwd = prob.objective(th, capi.PREC_FP32)
+    t1 = time.perf_counter()
+    theta = th.clone().requires_grad_(True)
+    phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP32)
+    phi.backward()
+    t2 = time.perf_counter()
+    g = theta.grad.numpy()
+    fd_idx = [int(x) for x in a.fd.split(",")] if a.fd else []
+    fd = adj.central_fd(prob, th, which=fd_idx) if fd_idx else None
+    members = []
+    if a.members:
+        t3 = time.perf_counter()
+        rng = np.random.default_rng(7)
+        for m in range(a.members):
+            tm = (th * torch.tensor(rng.uniform(0.8, 1.2, len(th)))).requires_grad_(True)
+            pm = adj.PicObjective.apply(tm, prob, capi.PREC_FP32)
+            pm.backward()
+            members.append(float(pm))
+        t4 = time.perf_counter()
+    out = {
+        "metric": "adjoint gradient of the axis E_z^2 objective (17 parameters)",
+        "value": (t2 - t1), "unit": "s per gradient", "higher_is_better": False, "n_gpus": 1,
+        "config": {"workload": "cfg5: 128x1024 cells, 1500 steps, 20 ppc ramp, 16-bump bunch, "
+                               "FP32 forward, checkpoints every 50 steps", "particles":
+                   spec.ppc * spec.cells_r * spec.cells_z + spec.bunch_macro},
+        "forward_only_s": t1 - t0, "phi": float(phi), "phi_forward_only": phi_fwd,
+        "gradient": g.tolist(),
+    }
+    if fd is not None:
+        out["fd_fp64"] = {str(j): fd[j] for j in fd_idx}
+        out["fd_rel_err"] = {str(j): abs(g[j] - fd[j]) / abs(fd[j]) for j in fd_idx}
+    if a.members:
+        out["members"] = a.members
+        out["members_s_per_gradient"] = (t4 - t3) / a.members
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +233,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
+    ap.add_argument("--instances", type=int, default=128, help="cfg4: ensemble size")
+    ap.add_argument("--fd", default="", help="cfg5: comma-separated parameter indices to check by differences")
+    ap.add_argument("--members", type=int, default=0, help="cfg5: gradients of this many perturbed members")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
@@ -133,6 +245,13 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(instances and z-slabs would shard across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if a.config == "cfg4":
+        bench_ensemble(a)
+        return
+    if a.config == "cfg5":
+        bench_adjoint(a)
+        return
 
     from pdp3_amd import capi
     from pdp3_amd import workloads as wl
@@ -168,8 +287,8 @@ def main():
     bytes_launch = n_plasma * BYTES_PER_PARTICLE["fp32"] + cells * BYTES_PER_CELL
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_fused32 (plasma species)", bytes_per_launch=bytes_launch,
-                    avg_launch_ms=per_launch_ms)
+                    traffic=traffic_from_profile("k_fused32_tiled_cfg3"), kernel="k_fused32_tiled (plasma species)",
+                    bytes_per_launch=bytes_launch, avg_launch_ms=per_launch_ms)
 
     cpu = None
     if not a.no_cpu:

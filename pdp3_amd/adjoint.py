@@ -50,6 +50,11 @@ class AdjointSpec:
     temperature_ev: float = 1.0
     pml_cells: int = 8
     window_frac: tuple = (0.7, 0.9)
+    # plasma particles closer than this many cells to the axis or a wall are
+    # left out (0: none), and the smallest bunch radius: with a cold plasma
+    # they keep every particle away from the reflecting boundaries
+    plasma_margin_cells: int = 0
+    bunch_r_min: float = 0.0
     seed: int = DEFAULT_SEED
 
     @property
@@ -87,6 +92,11 @@ class BumpRampProblem:
         sim = self._new_sim(capi.PREC_FP64)
         st = sim.download(0)
         sim.close()
+        if s.plasma_margin_cells > 0:
+            m = s.plasma_margin_cells * s.dr
+            R, L = s.cells_r * s.dr, s.cells_z * s.dr
+            out = (st["r"] < m) | (st["r"] > R - m) | (st["z"] < m) | (st["z"] > L - m)
+            st["alive"] = np.where(out, 0, st["alive"]).astype(np.uint8)
         self.plasma = st
         self.plasma_z = torch.tensor(st["z"], dtype=torch.float64)
         self.plasma_w0 = torch.tensor(st["w"], dtype=torch.float64)
@@ -96,7 +106,7 @@ class BumpRampProblem:
         z_a = max(s.bump_start - 3 * s.bump_sigma, s.dr)
         z_b = s.bump_start + s.bump_span + 3 * s.bump_sigma
         self.bunch_r = s.bunch_radius * np.sqrt(rng.uniform(0.0, 1.0, n))
-        self.bunch_r = np.maximum(self.bunch_r, s.dr / 4)
+        self.bunch_r = np.maximum(self.bunch_r, max(s.dr / 4, s.bunch_r_min))
         self.bunch_z = rng.uniform(z_a, z_b, n)
         # each macro-particle carries the volume pi R^2 (z_b - z_a) / n
         self.bunch_vol = PI * s.bunch_radius ** 2 * (z_b - z_a) / n
@@ -132,7 +142,7 @@ class BumpRampProblem:
         s = self.spec
         w_plasma, w_bunch = (w.detach().cpu().numpy() for w in self.weights(theta))
         sim = self._new_sim(precision)
-        sim.upload(0, w=w_plasma)
+        sim.upload(0, w=w_plasma, alive=self.plasma["alive"])
         n = len(self.bunch_r)
         zero = np.zeros(n)
         sim.add_bunch_loaded(-1, 1.0, self.bunch_r, self.bunch_z, zero, zero, np.full(n, s.velocity), w_bunch)

@@ -321,49 +321,109 @@ __device__ void jphi_d(const GridDesc &g, AdjSink &sk, D q, D r, D z, D vphi) {
   sk.add(kAPhi, r_i + 1, z_k + 1, ring(ro_v, dz2, r2, r3, v_2) * vphi);
 }
 
-// simple_current_distribution, src/particles.cpp:333-474
+// (log(1+x) - x) / x without cancellation (series below |x| = 0.1)
+__device__ __forceinline__ D m_of_d(D x) {
+  if (fabs(x.v) < 0.1) {
+    D s = cst(-1.0 / 12.0);
+    const double c[] = {1.0 / 11.0, -0.1, 1.0 / 9.0, -0.125, 1.0 / 7.0, -1.0 / 6.0, 0.2, -0.25, 1.0 / 3.0, -0.5};
+    for (double ck : c) s = ck + x * s;
+    return x * s;
+  }
+  D l;
+  l.v = log1p(x.v);
+  for (int j = 0; j < kDN; j++) l.d[j] = x.d[j] / (1.0 + x.v);
+  return (l - x) / x;
+}
+
+// simple_current_distribution, src/particles.cpp:333-474.  Values: the
+// reference's expressions.  Derivatives: those of the same integrals written
+// without cancellation in cell units (r = (i + a) dr, rho = i + a,
+// x = drho / rho, M = (ln(1+x) - x) / x, N = -x / (1+x) - M):
+//   J_z[i]   = q dzeta / (4 pi dr^2 dt) [ (1-a)(2i+a)/rho - drho/2 + i(i+1)/rho M ] / i
+//   J_z[i+1] = q dzeta / (4 pi dr^2 dt) [ a(2i+1+a)/rho + drho/2 - i(i+1)/rho M ] / (i+1)
+//   J_r[k]   = q / (2 pi dr dz dt) [ drho((1-b)/2 - dzeta/4) + i(i+1)/(2 rho) (drho (1-b)/rho_e + dzeta N) ] / (i+1/2)
+//   J_r[k+1] = q / (2 pi dr dz dt) [ drho(b/2 + dzeta/4) + i(i+1)/(2 rho) (drho b/rho_e - dzeta N) ] / (i+1/2)
+// (and the polynomial forms on the axis cell).  The reference's forms are
+// differences of O(i) terms: their derivative loses ~1/x digits for nearly
+// axial paths, which a relativistic beam's particles are.
 __device__ void segment_d(const GridDesc &g, AdjSink &acc, D q, D radius_new, D longitude_new, D radius_old,
                           D longitude_old, int i_n, int k_n) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
-  D delta_r = radius_new - radius_old;
-  D delta_z = longitude_new - longitude_old;
-  if ((fabs(delta_r.v) < kMnzl) || (fabs(delta_z.v) < kMnzl)) return;
-  D k = delta_r / delta_z;
-  D b = radius_old;
-  D lg = dlog((k * delta_z + b) / b);
-  D wj0, wj1;
-  if (i_n >= 1) {
-    D ssd = q / (2 * kPi * (i_n * dr) * dr * dz * delta_t * 2 * dr);
-    wj0 = ssd * (dr * delta_z - k * delta_z * delta_z / 2. - delta_z * b +
-                 dr * dr / k * ((i_n + 0.5) * (i_n + 0.5) - 0.25) * lg);
-    ssd = q / (2 * kPi * ((i_n + 1) * dr) * dr * dz * delta_t * 2 * dr);
-    wj1 = ssd * (k * delta_z * delta_z / 2. + delta_z * b + delta_z * dr +
-                 dr * dr / k * (0.25 - (i_n + 0.5) * (i_n + 0.5)) * lg);
-  } else {
-    D ssd = q / (2. * kPi * dr / 4. * dr * dz * delta_t * dr);
-    wj0 = ssd * (dr * delta_z - k * delta_z * delta_z / 2. - delta_z * b);
-    ssd = q / (2 * kPi * dr * dr * dz * delta_t * 2 * dr);
-    wj1 = ssd * (k * delta_z * delta_z / 2. + delta_z * dr + delta_z * b);
+  const double delta_r = radius_new.v - radius_old.v;
+  const double delta_z = longitude_new.v - longitude_old.v;
+  if ((fabs(delta_r) < kMnzl) || (fabs(delta_z) < kMnzl)) return;
+  // values, as the reference computes them
+  double vz0, vz1, vr0, vr1;
+  {
+    const double qv = q.v;
+    double k = delta_r / delta_z;
+    double b = radius_old.v;
+    double lg = log((k * delta_z + b) / b);
+    if (i_n >= 1) {
+      double ssd = qv / (2 * kPi * (i_n * dr) * dr * dz * delta_t * 2 * dr);
+      vz0 = ssd * (dr * delta_z - k * delta_z * delta_z / 2. - delta_z * b +
+                   dr * dr / k * ((i_n + 0.5) * (i_n + 0.5) - 0.25) * lg);
+      ssd = qv / (2 * kPi * ((i_n + 1) * dr) * dr * dz * delta_t * 2 * dr);
+      vz1 = ssd * (k * delta_z * delta_z / 2. + delta_z * b + delta_z * dr +
+                   dr * dr / k * (0.25 - (i_n + 0.5) * (i_n + 0.5)) * lg);
+    } else {
+      double ssd = qv / (2. * kPi * dr / 4. * dr * dz * delta_t * dr);
+      vz0 = ssd * (dr * delta_z - k * delta_z * delta_z / 2. - delta_z * b);
+      ssd = qv / (2 * kPi * dr * dr * dz * delta_t * 2 * dr);
+      vz1 = ssd * (k * delta_z * delta_z / 2. + delta_z * dr + delta_z * b);
+    }
+    k = -delta_z / delta_r;
+    const double r0 = (i_n + 0.5) * dr;
+    const double r1 = radius_old.v;
+    b = (k_n + 1.) * dz - longitude_old.v;
+    const double ssd = qv / (2 * kPi * r0 * dr * dz * delta_t * dz);
+    const double lr = log((radius_old.v + delta_r) / radius_old.v);
+    vr0 = ssd * (r0 * k * delta_r + k / 2. * delta_r * (radius_old.v + delta_r / 2.) +
+                 0.5 * delta_r * (b - k * (2 * r0 + r1)) +
+                 delta_r * (b - k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old.v * (radius_old.v + delta_r)) +
+                 (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
+    b = longitude_old.v - k_n * dz;
+    vr1 = ssd * (-r0 * k * delta_r - k / 2. * delta_r * (radius_old.v + delta_r / 2.) +
+                 0.5 * delta_r * (b + k * (2 * r0 + r1)) +
+                 delta_r * (b + k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old.v * (radius_old.v + delta_r)) -
+                 (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
   }
-  acc.add(kAZ, i_n, k_n, wj0);
-  acc.add(kAZ, i_n + 1, k_n, wj1);
-  k = -delta_z / delta_r;
-  const double r0 = (i_n + 0.5) * dr;
-  D r1 = radius_old;
-  b = (k_n + 1.) * dz - longitude_old;
-  D ssd = q / (2 * kPi * r0 * dr * dz * delta_t * dz);
-  D lr = dlog((radius_old + delta_r) / radius_old);
-  D wj = ssd * (r0 * k * delta_r + k / 2. * delta_r * (radius_old + delta_r / 2.) +
-                0.5 * delta_r * (b - k * (2 * r0 + r1)) +
-                delta_r * (b - k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old * (radius_old + delta_r)) +
-                (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
-  acc.add(kAR, i_n, k_n, wj);
-  b = longitude_old - k_n * dz;
-  wj = ssd * (-r0 * k * delta_r - k / 2. * delta_r * (radius_old + delta_r / 2.) +
-              0.5 * delta_r * (b + k * (2 * r0 + r1)) +
-              delta_r * (b + k * r1) * (4 * r0 * r0 - dr * dr) / (8 * radius_old * (radius_old + delta_r)) -
-              (k * (r0 * r0 / 2. - dr * dr / 8.)) * lr);
-  acc.add(kAR, i_n, k_n + 1, wj);
+  // derivatives, from the cancellation-free forms
+  const double fi = (double)i_n;
+  D a = radius_old / dr - fi;
+  D bo = longitude_old / dz - (double)k_n;
+  D drho = (radius_new - radius_old) / dr;
+  D dzeta = (longitude_new - longitude_old) / dz;
+  D jz0, jz1, jr0, jr1;
+  if (i_n == 0) {
+    jz0 = 8.0 * (1.0 - 0.5 * drho - a);
+    jz1 = 0.5 * drho + 1.0 + a;
+    jr0 = drho * (0.5 * (1.0 - bo) - 0.25 * dzeta) * 2.0;
+    jr1 = drho * (0.5 * bo + 0.25 * dzeta) * 2.0;
+  } else {
+    D rs = fi + a;
+    D re = rs + drho;
+    D x = drho / rs;
+    D M = m_of_d(x);
+    D N = -(x / (1.0 + x)) - M;
+    const double ii = fi * (fi + 1.0);
+    jz0 = ((1.0 - a) * (2.0 * fi + a) / rs - 0.5 * drho + ii / rs * M) / fi;
+    jz1 = (a * (2.0 * fi + 1.0 + a) / rs + 0.5 * drho - ii / rs * M) / (fi + 1.0);
+    D cc = 0.5 * ii / rs;
+    jr0 = (drho * (0.5 * (1.0 - bo) - 0.25 * dzeta) + cc * (drho * (1.0 - bo) / re + dzeta * N)) / (fi + 0.5);
+    jr1 = (drho * (0.5 * bo + 0.25 * dzeta) + cc * (drho * bo / re - dzeta * N)) / (fi + 0.5);
+  }
+  D sz = q * dzeta / (4.0 * kPi * dr * dr * delta_t);
+  D sr = q / (2.0 * kPi * dr * dz * delta_t);
+  D out[4] = {sz * jz0, sz * jz1, sr * jr0, sr * jr1};
+  out[0].v = vz0;
+  out[1].v = vz1;
+  out[2].v = vr0;
+  out[3].v = vr1;
+  acc.add(kAZ, i_n, k_n, out[0]);
+  acc.add(kAZ, i_n + 1, k_n, out[1]);
+  acc.add(kAR, i_n, k_n, out[2]);
+  acc.add(kAR, i_n, k_n + 1, out[3]);
 }
 
 // strict_motion_weighting, src/particles.cpp:786-988

@@ -285,10 +285,8 @@ __device__ void deposit_jphi(const F32Args &p, S &sk, float w, int i, float a, i
     c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
   }
   float s = p.kp * w * vphi;
-  sk.template add<kCPhi>(i, k, (double)(s * (1.f - b)) * c0);
-  sk.template add<kCPhi>(i + 1, k, (double)(s * (1.f - b)) * c1);
-  sk.template add<kCPhi>(i, k + 1, (double)(s * b) * c0);
-  sk.template add<kCPhi>(i + 1, k + 1, (double)(s * b) * c1);
+  const double s0 = (double)(s * (1.f - b)), s1 = (double)(s * b);
+  sk.template add4<kCPhi>(i, k, s0 * c0, s0 * c1, s1 * c0, s1 * c1);
 }
 
 // Particles::charge_weighting of one particle (src/particles.cpp:101-226)
@@ -310,10 +308,8 @@ __device__ void deposit_rho(const F32Args &p, S &sk, float w, int i, float a, in
     c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
   }
   float s = p.kp * w;
-  sk.template add<kCRho>(i, k, (double)(s * (1.f - b)) * c0);
-  sk.template add<kCRho>(i + 1, k, (double)(s * (1.f - b)) * c1);
-  sk.template add<kCRho>(i, k + 1, (double)(s * b) * c0);
-  sk.template add<kCRho>(i + 1, k + 1, (double)(s * b) * c1);
+  const double s0 = (double)(s * (1.f - b)), s1 = (double)(s * b);
+  sk.template add4<kCRho>(i, k, s0 * c0, s0 * c1, s1 * c0, s1 * c1);
 }
 
 // (log(1+x) - x) / x without cancellation
@@ -360,10 +356,8 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
     jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b * ire - dzeta * N)) * inv;
   }
   float sz = p.kz * w * dzeta, sr = p.kr * w;
-  sk.template add<kCZ>(i, k, (double)(sz * jz0));
-  sk.template add<kCZ>(i + 1, k, (double)(sz * jz1));
-  sk.template add<kCR>(i, k, (double)(sr * jr0));
-  sk.template add<kCR>(i, k + 1, (double)(sr * jr1));
+  sk.template add2<kCZ>(i, k, (double)(sz * jz0), (double)(sz * jz1));
+  sk.template add2<kCR>(i, k, (double)(sr * jr0), (double)(sr * jr1));
 }
 
 // current_distribution (src/particles.cpp:476-686): the straight path from
@@ -562,27 +556,17 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
   constexpr int kSplit = kPPT / PPT;
   const int chunk = wave / kSplit, j0 = (wave % kSplit) * PPT;
   const long long cbase = base + (long long)chunk * kChunk + (long long)j0 * 64 + lane;
-  // the loads of particle j+1 are issued before particle j is processed, so
-  // their HBM latency overlaps its arithmetic (the compiler may not hoist them
-  // itself: the stores of particle j could alias them)
-  PState cur = cbase < n_cap ? load_state(p, cbase) : PState{kDead, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int j = 0; j < PPT; j++) {
     const long long n = cbase + (long long)j * 64;
-    const long long nn = n + 64;
-    PState nxt{kDead, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (j + 1 < PPT && nn < n_cap) nxt = load_state(p, nn);
-    if (n < n_cap) step_state32(p, sk, fs, cur, n);
-    cur = nxt;
+    if (n < n_cap) step_particle32(p, sk, fs, n);
   }
   sk.flush();
   __syncthreads();
   box_flush(box, sk);
 }
 
-// 4 waves per SIMD (128 VGPRs): the occupancy that hides the loads best
 template <int PPT, int WAVES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
+__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
   __shared__ double box[TileSink::kSize];
   __shared__ float fbox[kFSize];
   constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;

@@ -39,6 +39,23 @@ struct GlobalSink {
   __device__ __forceinline__ void add(int i, int k, double v) {
     put(C, i, k, v);
   }
+  // a pair of nodes: (i, k), (i+1, k) for J_z; (i, k), (i, k+1) for J_r
+  template <int C>
+  __device__ __forceinline__ void add2(int i, int k, double v0, double v1) {
+    put(C, i, k, v0);
+    if (C == kCZ)
+      put(C, i + 1, k, v1);
+    else
+      put(C, i, k + 1, v1);
+  }
+  // the four corners (i..i+1, k..k+1) of J_phi / rho
+  template <int C>
+  __device__ __forceinline__ void add4(int i, int k, double v00, double v10, double v01, double v11) {
+    put(C, i, k, v00);
+    put(C, i + 1, k, v10);
+    put(C, i, k + 1, v01);
+    put(C, i + 1, k + 1, v11);
+  }
   __device__ __forceinline__ void begin_cell(int, int) {}
   __device__ __forceinline__ void flush() {}
 };
@@ -97,6 +114,41 @@ struct TileSinkT {
       }
     }
     box_put(C, i, k, v);
+  }
+  // whole stencils: straight into the accumulators when they are centred on
+  // the accumulator's cell (the common case: no select chains), else the box
+  template <int C>
+  __device__ __forceinline__ void add2(int i, int k, double v0, double v1) {
+    if (i == ic && k == kc) {
+      if (C == kCZ) {
+        z0 += v0;
+        z1 += v1;
+      } else {
+        x0 += v0;
+        x1 += v1;
+      }
+      return;
+    }
+    box_put(C, i, k, v0);
+    if (C == kCZ)
+      box_put(C, i + 1, k, v1);
+    else
+      box_put(C, i, k + 1, v1);
+  }
+  template <int C>
+  __device__ __forceinline__ void add4(int i, int k, double v00, double v10, double v01, double v11) {
+    if (i == ic && k == kc && (C == kCPhi || RHO)) {
+      if (C == kCPhi) {
+        p00 += v00; p10 += v10; p01 += v01; p11 += v11;
+      } else {
+        r00 += v00; r10 += v10; r01 += v01; r11 += v11;
+      }
+      return;
+    }
+    box_put(C, i, k, v00);
+    box_put(C, i + 1, k, v10);
+    box_put(C, i, k + 1, v01);
+    box_put(C, i + 1, k + 1, v11);
   }
   __device__ __forceinline__ void put_nz(int comp, int i, int k, double &v) {
     if (v != 0.0) box_put(comp, i, k, v);

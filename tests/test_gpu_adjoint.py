@@ -1,5 +1,11 @@
 """Adjoint gradients against central finite differences of the FP64 reference.
 
+Two problems.  The smooth one: a cold plasma kept three cells away from the
+axis and the walls and a bunch outside the innermost cell, so that no particle
+meets a reflecting boundary and the objective is a smooth function of the
+parameters: the adjoint must equal the difference quotients to 1e-4 relative.
+The realistic one (thermal plasma everywhere, bunch down to the axis):
+
 A reduced copy of the cfg5 problem (32 x 256 cells, 200 steps, checkpoints
 every 30 steps so that the last segment is partial, 16 bump amplitudes and
 the ramp slope).  The gradient of the axis E_z^2 objective comes from the
@@ -9,8 +15,16 @@ by the oracle, the deterministic CPU restatement of the reference loop: the
 GPU's FP64 form sums its deposits with atomics in a varying order, and the
 reference's J_r formula amplifies that round-off (to ~1e-9 relative per 100
 steps with a beam), which a 1e-4 step would turn into ~1e-4 noise.
-FP64 forward: 1e-4 relative on every component above 1% of the largest;
-FP32 forward (the production form): 2e-2 relative, its forward error.
+The reference loop is only piecewise smooth in the parameters: a particle
+reflected at the axis or a wall in one step instead of the next receives its
+E kick before instead of after the velocity flip, a jump of 2 q E dt / m.
+Central differences include the mean effect of such events, the adjoint (a
+derivative along the trajectory) does not; on this small problem the
+difference quotients themselves move by up to 1e-3 of the largest component
+between steps of 1e-3, 1e-4 and 1e-5.  The bar is therefore normwise:
+|g - fd|_inf / |fd|_inf <= 1e-3 for the FP64 forward and 2e-2 for the FP32
+forward (its forward error), and 1e-4 relative on the dominant components
+(above half of the largest), whose difference quotients converge.
 """
 import numpy as np
 import pytest
@@ -18,11 +32,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _problem():
+def _problem(smooth=False):
     from pdp3_amd import adjoint as adj
 
+    extra = dict(temperature_ev=0.0, plasma_margin_cells=3, bunch_r_min=5e-4) if smooth else {}
     spec = adj.cfg5_spec(cells_r=32, cells_z=256, steps=200, checkpoint_every=30, window_steps=60, ppc=4,
-                         bunch_macro=4000, window_frac=(0.2, 0.4))
+                         bunch_macro=4000, window_frac=(0.2, 0.4), **extra)
     return adj.BumpRampProblem(spec)
 
 
@@ -51,11 +66,8 @@ def oracle_objective(orc, prob, theta):
     return phi
 
 
-@pytest.fixture(scope="module")
-def problem_and_fd(orc):
-    prob = _problem()
+def _fd(orc, prob):
     th = prob.theta0()
-    print("adjoint: central differences of the oracle, 17 parameters", flush=True)
     fd = np.zeros(len(th))
     for j in range(len(th)):
         h = 1e-4 * abs(float(th[j]))
@@ -63,18 +75,65 @@ def problem_and_fd(orc):
         tp[j] += h
         tm[j] -= h
         fd[j] = (oracle_objective(orc, prob, tp) - oracle_objective(orc, prob, tm)) / (2 * h)
-    phi0 = oracle_objective(orc, prob, th)
-    return prob, th, fd, phi0
+    return prob, th, fd, oracle_objective(orc, prob, th)
 
 
-def _check(g, fd, tol):
+@pytest.fixture(scope="module")
+def problem_and_fd(orc):
+    print("adjoint: central differences of the oracle, 17 parameters", flush=True)
+    return _fd(orc, _problem())
+
+
+@pytest.fixture(scope="module")
+def smooth_problem_and_fd(orc):
+    print("adjoint (smooth problem): central differences of the oracle, 17 parameters", flush=True)
+    return _fd(orc, _problem(smooth=True))
+
+
+def test_adjoint_fp64_smooth_problem(smooth_problem_and_fd):
+    from pdp3_amd import adjoint as adj
+    from pdp3_amd import capi
+
+    prob, th, fd, phi0 = smooth_problem_and_fd
+    theta = th.clone().requires_grad_(True)
+    phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP64)
+    phi.backward()
+    print("phi gpu", float(phi), "oracle", phi0)
+    assert abs(float(phi) - phi0) <= 1e-6 * abs(phi0)
+    g = theta.grad.numpy()
     big = np.abs(fd) > 1e-2 * np.max(np.abs(fd))
     rel = np.abs(g[big] - fd[big]) / np.abs(fd[big])
     print("fd     ", np.array2string(fd, precision=9))
     print("adjoint", np.array2string(g, precision=9))
-    print("max rel", rel.max(), "over", big.sum(), "components")
-    assert big.sum() >= 5
-    assert rel.max() <= tol, rel
+    print("max rel", rel.max(), "over", big.sum())
+    assert big.sum() >= 10
+    assert rel.max() <= 1e-4, rel
+
+
+def test_adjoint_fp32_smooth_problem(smooth_problem_and_fd):
+    from pdp3_amd import adjoint as adj
+    from pdp3_amd import capi
+
+    prob, th, fd, phi0 = smooth_problem_and_fd
+    theta = th.clone().requires_grad_(True)
+    phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP32)
+    phi.backward()
+    print("phi fp32", float(phi), "oracle", phi0)
+    _check(theta.grad.numpy(), fd, 2e-2, 2e-2)
+
+
+def _check(g, fd, tol_norm, tol_dominant):
+    scale = np.max(np.abs(fd))
+    norm = np.max(np.abs(g - fd)) / scale
+    rel = np.abs(g - fd) / np.abs(fd)
+    dom = np.abs(fd) >= 0.5 * scale
+    print("fd     ", np.array2string(fd, precision=9))
+    print("adjoint", np.array2string(g, precision=9))
+    print("relative per component", np.array2string(rel, precision=2))
+    print("normwise", norm, "dominant max rel", rel[dom].max(), "over", dom.sum())
+    assert np.all(np.sign(g[np.abs(fd) > 0.05 * scale]) == np.sign(fd[np.abs(fd) > 0.05 * scale]))
+    assert norm <= tol_norm, norm
+    assert rel[dom].max() <= tol_dominant, rel[dom]
 
 
 def test_adjoint_fp64_matches_fd(problem_and_fd):
@@ -87,7 +146,7 @@ def test_adjoint_fp64_matches_fd(problem_and_fd):
     phi.backward()
     print("phi gpu", float(phi), "oracle", phi0)
     assert abs(float(phi) - phi0) <= 1e-6 * abs(phi0)
-    _check(theta.grad.numpy(), fd, 1e-4)
+    _check(theta.grad.numpy(), fd, 1e-3, 1e-4)
 
 
 def test_adjoint_fp32_matches_fd(problem_and_fd):
@@ -99,4 +158,4 @@ def test_adjoint_fp32_matches_fd(problem_and_fd):
     phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP32)
     phi.backward()
     print("phi fp32", float(phi), "oracle", phi0)
-    _check(theta.grad.numpy(), fd, 2e-2)
+    _check(theta.grad.numpy(), fd, 2e-2, 2e-2)
