@@ -55,6 +55,9 @@ class AdjointSpec:
     # they keep every particle away from the reflecting boundaries
     plasma_margin_cells: int = 0
     bunch_r_min: float = 0.0
+    # spread (m/s, uniform in +-) of the bunch's initial radial velocity; 0
+    # starts every bunch particle on a purely axial path
+    bunch_vr_spread: float = 0.0
     seed: int = DEFAULT_SEED
 
     @property
@@ -108,6 +111,7 @@ class BumpRampProblem:
         self.bunch_r = s.bunch_radius * np.sqrt(rng.uniform(0.0, 1.0, n))
         self.bunch_r = np.maximum(self.bunch_r, max(s.dr / 4, s.bunch_r_min))
         self.bunch_z = rng.uniform(z_a, z_b, n)
+        self.bunch_vr = rng.uniform(-1.0, 1.0, n) * s.bunch_vr_spread
         # each macro-particle carries the volume pi R^2 (z_b - z_a) / n
         self.bunch_vol = PI * s.bunch_radius ** 2 * (z_b - z_a) / n
         centres = s.bump_start + np.arange(s.n_bumps) * (s.bump_span / max(s.n_bumps - 1, 1))
@@ -145,7 +149,7 @@ class BumpRampProblem:
         sim.upload(0, w=w_plasma, alive=self.plasma["alive"])
         n = len(self.bunch_r)
         zero = np.zeros(n)
-        sim.add_bunch_loaded(-1, 1.0, self.bunch_r, self.bunch_z, zero, zero, np.full(n, s.velocity), w_bunch)
+        sim.add_bunch_loaded(-1, 1.0, self.bunch_r, self.bunch_z, self.bunch_vr, zero, np.full(n, s.velocity), w_bunch)
         return sim
 
     def objective(self, theta, precision=capi.PREC_FP64):
