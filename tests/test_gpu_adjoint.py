@@ -24,7 +24,8 @@ stays put); the adjoint, a derivative along the trajectory, does not.  Hence:
     spread): 1e-4 normwise and on the components above a quarter of the
     largest, FP64 forward; 2e-2 for the FP32 forward (its forward error);
   * the realistic problem (plasma down to the axis, a purely axial bunch):
-    1e-3 normwise, FP64 forward, and 2e-2 for the FP32 forward.
+    1e-3 normwise (1e-2 per dominant component), FP64 forward, and 2e-2 for
+    the FP32 forward.
 """
 import numpy as np
 import pytest
@@ -139,7 +140,7 @@ def test_adjoint_fp64_matches_fd(problem_and_fd):
     phi.backward()
     print("phi gpu", float(phi), "oracle", phi0)
     assert abs(float(phi) - phi0) <= 1e-6 * abs(phi0)
-    _check(theta.grad.numpy(), fd, 1e-3, 1e-3)
+    _check(theta.grad.numpy(), fd, 1e-3, 1e-2)
 
 
 def test_adjoint_fp32_matches_fd(problem_and_fd):
