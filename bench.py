@@ -180,19 +180,28 @@ def bench_adjoint(a):
     from pdp3_amd import adjoint as adj
     from pdp3_amd import capi
 
-    spec = adj.cfg5_spec()
+    spec = adj.cfg5_spec(steps=a.adj_steps, window_steps=min(300, a.adj_steps)) if a.adj_steps else adj.cfg5_spec()
     prob = adj.BumpRampProblem(spec)
     th = prob.theta0()
+    log = lambda *m: print("#", *m, file=sys.stderr, flush=True)
+    log(f"cfg5: {spec.cells_r}x{spec.cells_z} cells, {spec.steps} steps")
     t0 = time.perf_counter()
     phi_fwd = prob.objective(th, capi.PREC_FP32)
     t1 = time.perf_counter()
+    log(f"forward only: {t1 - t0:.2f} s, phi {phi_fwd}")
     theta = th.clone().requires_grad_(True)
     phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP32)
     phi.backward()
     t2 = time.perf_counter()
+    log(f"gradient: {t2 - t1:.2f} s")
     g = theta.grad.numpy()
     fd_idx = [int(x) for x in a.fd.split(",")] if a.fd else []
-    fd = adj.central_fd(prob, th, which=fd_idx) if fd_idx else None
+    fd = None
+    if fd_idx:
+        fd = np.zeros(len(th))
+        for j in fd_idx:
+            fd[j] = adj.central_fd(prob, th, which=[j])[j]
+            log(f"fd[{j}] = {fd[j]:.6e}, adjoint {g[j]:.6e}")
     members = []
     if a.members:
         t3 = time.perf_counter()
@@ -202,6 +211,7 @@ def bench_adjoint(a):
             pm = adj.PicObjective.apply(tm, prob, capi.PREC_FP32)
             pm.backward()
             members.append(float(pm))
+            log(f"member {m}: phi {float(pm):.6e}")
         t4 = time.perf_counter()
     out = {
         "metric": "adjoint gradient of the axis E_z^2 objective (17 parameters)",
@@ -236,6 +246,7 @@ def main():
     ap.add_argument("--instances", type=int, default=128, help="cfg4: ensemble size")
     ap.add_argument("--fd", default="", help="cfg5: comma-separated parameter indices to check by differences")
     ap.add_argument("--members", type=int, default=0, help="cfg5: gradients of this many perturbed members")
+    ap.add_argument("--adj-steps", type=int, default=0, help="cfg5: override the step count (default 1500)")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
