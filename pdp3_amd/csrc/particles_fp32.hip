@@ -27,9 +27,12 @@ namespace {
 
 constexpr uint32_t kDead = 0x80000000u;
 
-// a / b through the hardware reciprocal (about 1 ulp; inputs are never
-// denormal here)
+// a / b, 1/sqrt(x) and sqrt(x) through the hardware instructions (about 1
+// ulp each; no denormal arguments reach them here: the Lorentz factors are
+// >= 1, the other arguments are O(1) in cell units or exactly 0)
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ int cell_i(uint32_t c) { return (int)((c >> 16) & 0x7FFF); }
 __device__ __forceinline__ int cell_k(uint32_t c) { return (int)(c & 0xFFFF); }
@@ -64,7 +67,10 @@ struct RadW {
   int ir;
   float A, B;
 };
-__device__ __forceinline__ RadW rad_lo(int i, float a) {
+// both radial weight pairs of a particle in cell i at offset a.  The four
+// denominators 2(l+1), 2(l+3) (lo family, l = its staggered index) and 2i,
+// 2(i+2) (own family) share one hardware reciprocal of their product.
+__device__ __forceinline__ void rad_weights(int i, float a, RadW &lo, RadW &own) {
   int ir = i;
   float al = a - 0.5f;
   if (al <= 0.f) {
@@ -75,26 +81,24 @@ __device__ __forceinline__ RadW rad_lo(int i, float a) {
     ir = 0;
     al = (float)i + a - 0.5f;
   }
-  float fi = (float)ir;
-  RadW w;
-  w.ir = ir;
-  w.A = fdiv((1.f - al) * (2.f * fi + 1.f + al), 2.f * (fi + 1.f));
-  w.B = fdiv(al * (2.f * fi + 2.f + al), 2.f * (fi + 3.f));
-  return w;
-}
-__device__ __forceinline__ RadW rad_own(int i, float a) {
-  RadW w;
-  w.ir = i;
+  const float fl = (float)ir, fi = (float)i;
+  const float d1 = 2.f * (fl + 1.f), d3 = 2.f * (fl + 3.f);
+  const float e0 = i == 0 ? 1.f : 2.f * fi, e2 = 2.f * (fi + 2.f);
+  const float p13 = d1 * d3, p02 = e0 * e2;
+  const float inv = __builtin_amdgcn_rcpf(p13 * p02);
+  const float i13 = p02 * inv, i02 = p13 * inv;  // 1 / (d1 d3), 1 / (e0 e2)
+  lo.ir = ir;
+  lo.A = (1.f - al) * (2.f * fl + 1.f + al) * (d3 * i13);
+  lo.B = al * (2.f * fl + 2.f + al) * (d1 * i13);
+  own.ir = i;
   if (i == 0) {
     double ad = a;  // axis cell in FP64: CYL_VOL annulus weight
-    w.A = (float)((1.0 - ad) * ad * 4.0);
-    w.B = (float)(ad * (1.0 + ad) / 4.0);
+    own.A = (float)((1.0 - ad) * ad * 4.0);
+    own.B = (float)(ad * (1.0 + ad) / 4.0);
   } else {
-    float fi = (float)i;
-    w.A = fdiv((1.f - a) * (2.f * fi + a), 2.f * fi);
-    w.B = fdiv(a * (2.f * fi + 1.f + a), 2.f * (fi + 2.f));
+    own.A = (1.f - a) * (2.f * fi + a) * (e2 * i02);
+    own.B = a * (2.f * fi + 1.f + a) * (e0 * i02);
   }
-  return w;
 }
 struct AxW {
   int kz;
@@ -216,7 +220,7 @@ __device__ __forceinline__ void boris_u(const F32Args &p, const float e[3], cons
   float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
   float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
   for (int j = 0; j < 3; j++) u[j] += ee[j];
-  float g = rsqrtf(1.f + dot3f(u, u) * p.inv_c2);
+  float g = frsq(1.f + dot3f(u, u) * p.inv_c2);
   for (int j = 0; j < 3; j++) b[j] *= g;
   float c2 = fdiv(2.f, 1.f + dot3f(b, b));
   float t0 = u[0], t1 = u[1], t2 = u[2];
@@ -231,7 +235,7 @@ __device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const 
   float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
   float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
   float t[3] = {u[0], u[1], u[2]};
-  float g = rsqrtf(1.f + dot3f(t, t) * p.inv_c2);
+  float g = frsq(1.f + dot3f(t, t) * p.inv_c2);
   float up[3];
   up[0] = t[0] + ee[0] + g * (t[1] * b[2] - t[2] * b[1]);
   up[1] = t[1] + ee[1] + g * (t[2] * b[0] - t[0] * b[2]);
@@ -241,7 +245,7 @@ __device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const 
   float s = alpha - B2;
   float ub = dot3f(up, b);
   float us2 = ub * ub;
-  float ig = rsqrtf(0.5f * (s + sqrtf(s * s + 4.f * (B2 + us2))));  // 1/gamma^{n+1}
+  float ig = frsq(0.5f * (s + fsqrt(s * s + 4.f * (B2 + us2))));  // 1/gamma^{n+1}
   for (int j = 0; j < 3; j++) b[j] *= ig;
   s = fdiv(1.f, 1.f + dot3f(b, b));
   alpha = dot3f(up, b);
@@ -262,12 +266,17 @@ __device__ __forceinline__ void renorm(int &c, float &o) {
 }
 
 // r of the Cartesian point (X, dy), X = i + xa, as an offset update that
-// never forms X^2 + dy^2 - X^2: sqrt(X^2+dy^2) - X = dy^2 / (R + X)
-__device__ __forceinline__ float rotate_back(int i, float &xa, float dy) {
-  float X = (float)i + xa;
-  float R = sqrtf(X * X + dy * dy);
-  xa += fdiv(dy * dy, R + X);
-  return fdiv(dy, R);  // sin(theta)
+// never forms X^2 + dy^2 - X^2: R - X = dy^2 / (R + X) for X > 0 (and
+// R + |X| when the point lies behind the axis).  Returns sin(theta) = dy / R;
+// ct = |X| / R is the reference's sqrt(1 - sin^2) (src/particles.cpp:1010).
+__device__ __forceinline__ float rotate_back(int i, float &xa, float dy, float &ct) {
+  const float X = (float)i + xa;
+  const float s = fmaxf(X * X + dy * dy, 1e-30f);
+  const float q = frsq(s);  // 1 / R
+  const float R = s * q;
+  xa += X > 0.f ? fdiv(dy * dy, R + X) : R - X;
+  ct = fabsf(X) * q;
+  return dy * q;
 }
 
 // J_phi of one particle at the half-step position,
@@ -280,9 +289,11 @@ __device__ void deposit_jphi(const F32Args &p, S &sk, float w, int i, float a, i
     c0 = 8.0 * (1.0 - ad);
     c1 = 1.0 + ad;
   } else {
-    float fi = (float)i, rho = fi + a;
-    c0 = fdiv((1.f - a) * (2.f * fi + a), rho * fi);
-    c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
+    // 1 / (rho i) and 1 / (rho (i+1)) from one reciprocal
+    const float fi = (float)i, rho = fi + a, fi1 = fi + 1.f;
+    const float inv = __builtin_amdgcn_rcpf(rho * fi * fi1);
+    c0 = (1.f - a) * (2.f * fi + a) * (fi1 * inv);
+    c1 = a * (2.f * fi + 1.f + a) * (fi * inv);
   }
   float s = p.kp * w * vphi;
   const double s0 = (double)(s * (1.f - b)), s1 = (double)(s * b);
@@ -303,9 +314,10 @@ __device__ void deposit_rho(const F32Args &p, S &sk, float w, int i, float a, in
       c1 = 1.0 + rho;
     }
   } else {
-    float fi = (float)i, rho = fi + a;
-    c0 = fdiv((1.f - a) * (2.f * fi + a), rho * fi);
-    c1 = fdiv(a * (2.f * fi + 1.f + a), rho * (fi + 1.f));
+    const float fi = (float)i, rho = fi + a, fi1 = fi + 1.f;
+    const float inv = __builtin_amdgcn_rcpf(rho * fi * fi1);
+    c0 = (1.f - a) * (2.f * fi + a) * (fi1 * inv);
+    c1 = a * (2.f * fi + 1.f + a) * (fi * inv);
   }
   float s = p.kp * w;
   const double s0 = (double)(s * (1.f - b)), s1 = (double)(s * b);
@@ -340,18 +352,20 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
     jr0 = drho * (0.5f * (1.f - b) - 0.25f * dzeta) * 2.f;
     jr1 = drho * (0.5f * b + 0.25f * dzeta) * 2.f;
   } else {
-    float fi = (float)i;
-    float rs = fi + a, re = rs + drho;
-    float irs = __builtin_amdgcn_rcpf(rs);
+    const float fi = (float)i, fi1 = fi + 1.f, fh = fi + 0.5f;
+    const float rs = fi + a, re = rs + drho;
+    // 1/rs, 1/re from one reciprocal; 1/i, 1/(i+1), 1/(i+1/2) from another
+    const float q = __builtin_amdgcn_rcpf(rs * re);
+    const float irs = re * q, ire = rs * q;
+    const float ii = fi * fi1;
+    const float qi = __builtin_amdgcn_rcpf(ii * fh);
+    const float inv_i = fi1 * fh * qi, inv_i1 = fi * fh * qi, inv = ii * qi;
     float x = drho * irs;
     float M = m_of(x);
-    float N = -fdiv(x, 1.f + x) - M;
-    float ii = fi * (fi + 1.f);
-    jz0 = fdiv((1.f - a) * (2.f * fi + a) * irs - 0.5f * drho + ii * irs * M, fi);
-    jz1 = fdiv(a * (2.f * fi + 1.f + a) * irs + 0.5f * drho - ii * irs * M, fi + 1.f);
+    float N = -drho * ire - M;  // x / (1 + x) = drho / re
+    jz0 = ((1.f - a) * (2.f * fi + a) * irs - 0.5f * drho + ii * irs * M) * inv_i;
+    jz1 = (a * (2.f * fi + 1.f + a) * irs + 0.5f * drho - ii * irs * M) * inv_i1;
     float cc = 0.5f * ii * irs;
-    float inv = __builtin_amdgcn_rcpf(fi + 0.5f);
-    float ire = __builtin_amdgcn_rcpf(re);
     jr0 = (drho * (0.5f * (1.f - b) - 0.25f * dzeta) + cc * (drho * (1.f - b) * ire + dzeta * N)) * inv;
     jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b * ire - dzeta * N)) * inv;
   }
@@ -458,21 +472,36 @@ struct PState {
   float a, b, u0, u1, u2, w;
 };
 
-__device__ __forceinline__ PState load_state(const F32Args &p, long long n) {
+// the 4-byte element at byte offset `off` from p: a 32-bit offset from a
+// uniform base is one VGPR and scalar-base addressing, where an element index
+// would be widened to a 64-bit address per array
+template <class T>
+__device__ __forceinline__ T &at(T *p, unsigned off) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + off);
+}
+template <class T>
+__device__ __forceinline__ const T &at(const T *p, unsigned off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + off);
+}
+
+__device__ __forceinline__ PState load_state(const F32Args &p, unsigned off) {
   PState s;
-  s.c = p.cell[n];
-  s.a = p.fa[n];
-  s.b = p.fb[n];
-  s.u0 = p.ur[n];
-  s.u1 = p.uphi[n];
-  s.u2 = p.uz[n];
-  s.w = p.w[n];
+  s.c = at(p.cell, off);
+  s.a = at(p.fa, off);
+  s.b = at(p.fb, off);
+  s.u0 = at(p.ur, off);
+  s.u1 = at(p.uphi, off);
+  s.u2 = at(p.uz, off);
+  s.w = at(p.w, off);
+  // all seven loads are issued together, ahead of the dead-particle branch
+  // (instead of the six that the branch guards waiting for the cell word)
+  asm volatile("" ::"v"(s.c), "v"(s.a), "v"(s.b), "v"(s.u0), "v"(s.u1), "v"(s.u2), "v"(s.w));
   return s;
 }
 
-// the whole step of one particle (state st, slot n)
+// the whole step of one particle (state ps, slot at byte offset off)
 template <class S, class FS>
-__device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &fs, const PState &ps, long long n) {
+__device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &fs, const PState &ps, unsigned off) {
   const uint32_t c = ps.c;
   if (c & kDead) return;
   int i = cell_i(c), k = cell_k(c);
@@ -481,7 +510,8 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
   const float w = ps.w;
   sk.begin_cell(i, k);
   // gathers (E at the full step, H at the half-time copy)
-  RadW rl = rad_lo(i, a), ro = rad_own(i, a);
+  RadW rl, ro;
+  rad_weights(i, a, rl, ro);
   AxW zo = ax_own(k, b), zh = ax_half(k, b);
   float e[3], h[3];
   fs.get(p, i, k, rl, ro, zo, zh, e, h);
@@ -489,12 +519,13 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
     vay_u(p, e, h, u);
   else
     boris_u(p, e, h, u);
-  float g = rsqrtf(1.f + dot3f(u, u) * p.inv_c2);
+  float g = frsq(1.f + dot3f(u, u) * p.inv_c2);
   float dx = u[0] * g * p.hr, dy = u[1] * g * p.hr, dz = u[2] * g * p.hz;
   // first half step and rotation back to r-z (step_v, :1296-1316)
   int i1 = i, k1 = k;
   float xa = a + dx, b1 = b + dz;
-  rotate_back(i1, xa, dy);
+  float ct;
+  rotate_back(i1, xa, dy, ct);
   renorm(i1, xa);
   renorm(k1, b1);
   deposit_jphi(p, sk, w, i1, xa, k1, b1, u[1] * g);
@@ -502,32 +533,49 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
   float xa2 = xa + dx, b2 = b1 + dz;
   bool alive = true;
   walls(p, i1, xa2, k1, b2, u, alive);
-  float st = rotate_back(i1, xa2, dy);
+  const float st = rotate_back(i1, xa2, dy, ct);
   renorm(i1, xa2);
   renorm(k1, b2);
   if (alive) {
     deposit_j(p, sk, w, i, a, k, b, i1, xa2, k1, b2);
     // back_velocity_to_rz with the second half step's angle (:1005-1023)
-    float ct = sqrtf(1.f - st * st);
     float u0 = ct * u[0] - st * u[1];
     float u1 = st * u[0] + ct * u[1];
     u[0] = u0;
     u[1] = u1;
     if (p.want_rho) deposit_rho(p, sk, w, i1, xa2, k1, b2);
   }
-  p.cell[n] = pack(i1, k1) | (alive ? 0u : kDead);
-  p.fa[n] = xa2;
-  p.fb[n] = b2;
-  p.ur[n] = u[0];
-  p.uphi[n] = u[1];
-  p.uz[n] = u[2];
+  // a fresh copy of the offset: the stores then address from the scalar bases
+  // too, instead of reusing 64-bit addresses formed for the loads
+  unsigned so = off;
+  asm("" : "+v"(so));
+  at(p.cell, so) = pack(i1, k1) | (alive ? 0u : kDead);
+  at(p.fa, so) = xa2;
+  at(p.fb, so) = b2;
+  at(p.ur, so) = u[0];
+  at(p.uphi, so) = u[1];
+  at(p.uz, so) = u[2];
 }
 
-// chunked plasma species: 4 wavefronts per block, each walking one chunk of
-// kChunk slots kPPT particles per lane, deposits through the tile sink
+// the whole step of the particle in slot `slot` of the arrays of p (which
+// start at most 2^30 slots before it)
 template <class S, class FS>
-__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, long long n) {
-  step_state32(p, sk, fs, load_state(p, n), n);
+__device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, unsigned slot) {
+  step_state32(p, sk, fs, load_state(p, slot * 4u), slot * 4u);
+}
+
+// the arguments with the particle arrays starting at slot `base` (uniform over
+// the block): slots are then addressed by a 32-bit offset from a scalar base
+__device__ __forceinline__ F32Args from_slot(const F32Args &p, long long base) {
+  F32Args q = p;
+  q.cell += base;
+  q.fa += base;
+  q.fb += base;
+  q.ur += base;
+  q.uphi += base;
+  q.uz += base;
+  q.w += base;
+  return q;
 }
 
 // one tile of a cell-sorted (chunked) species: CB = WAVES * PPT / kPPT chunks
@@ -555,10 +603,14 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int kSplit = kPPT / PPT;
   const int chunk = wave / kSplit, j0 = (wave % kSplit) * PPT;
-  const long long cbase = base + (long long)chunk * kChunk + (long long)j0 * 64 + lane;
+  constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
+  const F32Args q = from_slot(p, base);
+  const unsigned lim = (unsigned)min(n_cap - base, kTile);
+  const unsigned off0 = (unsigned)(chunk * kChunk + j0 * 64 + lane);
   for (int j = 0; j < PPT; j++) {
-    const long long n = cbase + (long long)j * 64;
-    if (n < n_cap) step_particle32(p, sk, fs, n);
+    unsigned n = off0 + (unsigned)j * 64u;
+    asm("" : "+v"(n));  // keeps the slot a 32-bit offset from the scalar bases (no 64-bit pointer per array)
+    if (n < lim) step_particle32(q, sk, fs, n);
   }
   sk.flush();
   __syncthreads();
@@ -566,7 +618,7 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
 }
 
 template <int PPT, int WAVES>
-__global__ void __launch_bounds__(256) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
   __shared__ double box[TileSink::kSize];
   __shared__ float fbox[kFSize];
   constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
@@ -599,9 +651,12 @@ __device__ __forceinline__ void fused32_bunch_tile(const F32Args &p, const Globa
   BunchSink sk;
   box_init(box, ai, ak, sk, gs);
   __syncthreads();
+  const F32Args q = from_slot(p, base);
+  const unsigned lim = (unsigned)min(n_cap - base, (long long)WAVES * kChunk);
+  const unsigned off0 = (unsigned)(wave * kChunk + lane);
   for (int j = 0; j < kPPT; j++) {
-    long long n = cbase + (long long)j * 64;
-    if (n < n_cap) step_particle32(p, sk, GlobalFields(), n);
+    const unsigned n = off0 + (unsigned)j * 64u;
+    if (n < lim) step_particle32(q, sk, GlobalFields(), n);
   }
   sk.flush();
   __syncthreads();
@@ -642,10 +697,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 
 // plain layout (bunches): one particle per thread, deposits straight to HBM
 __global__ void __launch_bounds__(256) k_fused32_plain(F32Args p, GlobalSink gs, long long n_cap) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= n_cap) return;
+  const long long base = (long long)blockIdx.x * blockDim.x;
+  if (base + threadIdx.x >= n_cap) return;
   GlobalSink sk = gs;
-  step_particle32(p, sk, GlobalFields(), n);
+  step_particle32(from_slot(p, base), sk, GlobalFields(), threadIdx.x);
 }
 
 __global__ void k_charge32(F32Args p, GlobalSink gs, long long n_cap) {
