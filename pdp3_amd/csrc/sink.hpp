@@ -150,26 +150,52 @@ struct TileSinkT {
     box_put(C, i, k + 1, v01);
     box_put(C, i + 1, k + 1, v11);
   }
-  __device__ __forceinline__ void put_nz(int comp, int i, int k, double &v) {
-    if (v != 0.0) box_put(comp, i, k, v);
-    v = 0.0;
+  __device__ __forceinline__ void put_nz(int comp, int i, int k, double v) {
+    if (v != 0.0) g.put(comp, i, k, v);
   }
+  // the accumulators' stencil (nodes ic..ic+1, kc..kc+1) in one piece: when
+  // it lies inside the box, one bounds check and LDS atomics at fixed offsets
+  // from one address (zeros included: cheaper than testing each value);
+  // otherwise node by node to HBM
   __device__ __forceinline__ void flush() {
     if (ic < 0) return;
-    put_nz(kCPhi, ic, kc, p00);
-    put_nz(kCPhi, ic + 1, kc, p10);
-    put_nz(kCPhi, ic, kc + 1, p01);
-    put_nz(kCPhi, ic + 1, kc + 1, p11);
-    if (RHO) {
-      put_nz(kCRho, ic, kc, r00);
-      put_nz(kCRho, ic + 1, kc, r10);
-      put_nz(kCRho, ic, kc + 1, r01);
-      put_nz(kCRho, ic + 1, kc + 1, r11);
+    const int li = ic - bi, lk = kc - bk;
+    if ((unsigned)li < (unsigned)(BR - 1) && (unsigned)lk < (unsigned)(BK - 1)) {
+      double *q = box + li * BK + lk;
+      constexpr int C = BR * BK;
+      atomicAdd(q + kCPhi * C, p00);
+      atomicAdd(q + kCPhi * C + BK, p10);
+      atomicAdd(q + kCPhi * C + 1, p01);
+      atomicAdd(q + kCPhi * C + BK + 1, p11);
+      if (RHO) {
+        atomicAdd(q + kCRho * C, r00);
+        atomicAdd(q + kCRho * C + BK, r10);
+        atomicAdd(q + kCRho * C + 1, r01);
+        atomicAdd(q + kCRho * C + BK + 1, r11);
+      }
+      atomicAdd(q + kCZ * C, z0);
+      atomicAdd(q + kCZ * C + BK, z1);
+      atomicAdd(q + kCR * C, x0);
+      atomicAdd(q + kCR * C + 1, x1);
+    } else {
+      put_nz(kCPhi, ic, kc, p00);
+      put_nz(kCPhi, ic + 1, kc, p10);
+      put_nz(kCPhi, ic, kc + 1, p01);
+      put_nz(kCPhi, ic + 1, kc + 1, p11);
+      if (RHO) {
+        put_nz(kCRho, ic, kc, r00);
+        put_nz(kCRho, ic + 1, kc, r10);
+        put_nz(kCRho, ic, kc + 1, r01);
+        put_nz(kCRho, ic + 1, kc + 1, r11);
+      }
+      put_nz(kCZ, ic, kc, z0);
+      put_nz(kCZ, ic + 1, kc, z1);
+      put_nz(kCR, ic, kc, x0);
+      put_nz(kCR, ic, kc + 1, x1);
     }
-    put_nz(kCZ, ic, kc, z0);
-    put_nz(kCZ, ic + 1, kc, z1);
-    put_nz(kCR, ic, kc, x0);
-    put_nz(kCR, ic, kc + 1, x1);
+    p00 = p10 = p01 = p11 = 0.0;
+    r00 = r10 = r01 = r11 = 0.0;
+    z0 = z1 = x0 = x1 = 0.0;
   }
   // the next particle's contributions are centred on cell (i, k)
   __device__ __forceinline__ void begin_cell(int i, int k) {
