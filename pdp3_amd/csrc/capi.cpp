@@ -293,6 +293,19 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
   c.sigma_host.assign(n, 0.0);
   c.eps_host.assign(n, 1.0);  // Geometry::set_epsilon, src/geometry.cpp:33-39
   if ((st = upload_koef(ctx))) return bail(st);
+  {
+    std::vector<float4> rows(2 * (size_t)n_grid_r);
+    for (int i = 0; i < n_grid_r; i++) {
+      const double fi = i;
+      const float r0 = i ? (float)(1.0 / (2.0 * fi)) : 0.f, i0 = i ? (float)(1.0 / fi) : 0.f;
+      rows[2 * i] = make_float4(r0, (float)(1.0 / (2.0 * (fi + 1))), (float)(1.0 / (2.0 * (fi + 2))),
+                                (float)(1.0 / (2.0 * (fi + 3))));
+      rows[2 * i + 1] = make_float4(i0, (float)(1.0 / (fi + 1)), (float)(1.0 / (fi + 0.5)), (float)(fi * (fi + 1)));
+    }
+    if ((st = dalloc(ctx, &c.rows, rows.size()))) return bail(st);
+    if (hipMemcpy(c.rows, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(PDP3_ERR_HIP);
+  }
   *out = ctx;
   return PDP3_OK;
 }
@@ -311,6 +324,7 @@ int pdp3_ctx_destroy(pdp3_ctx *ctx) {
   dfree(c.sort_ws);
   dfree(c.rho_all);
   dfree(c.gauss);
+  dfree(c.rows);
   if (c.err_host) (void)hipHostFree(c.err_host);
   for (auto &t : c.timers) {
     for (auto &ev : t.pending) {

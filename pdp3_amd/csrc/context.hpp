@@ -143,6 +143,10 @@ struct Ctx {
   // Gauss-law diagnostic: charge density of all species and the residual
   double *rho_all = nullptr;
   double *gauss = nullptr;
+  // FP32 form: per radial cell index i (0 .. nr-1) two float4 entries,
+  // {1/(2i), 1/(2(i+1)), 1/(2(i+2)), 1/(2(i+3))} and {1/i, 1/(i+1), 1/(i+1/2),
+  // i(i+1)}, with 1/0 stored as 0 (built at context creation)
+  float4 *rows = nullptr;
   size_t cells() const { return (size_t)g.nr * (size_t)g.nz; }
 };
 

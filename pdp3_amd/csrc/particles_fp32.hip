@@ -51,6 +51,9 @@ struct F32Args {
   int want_rho;
   int n;
   const double *er, *ephi, *ez, *hhr, *hhphi, *hhz;
+  // per radial cell index i, two entries: {1/(2i), 1/(2(i+1)), 1/(2(i+2)),
+  // 1/(2(i+3))} and {1/i, 1/(i+1), 1/(i+1/2), i(i+1)} (1/0 read as 0)
+  const float4 *rows;
   int *err;
   uint32_t *cell;
   float *fa, *fb, *ur, *uphi, *uz;
@@ -67,37 +70,36 @@ struct RadW {
   int ir;
   float A, B;
 };
-// both radial weight pairs of a particle in cell i at offset a.  The four
-// denominators 2(l+1), 2(l+3) (lo family, l = its staggered index) and 2i,
-// 2(i+2) (own family) share one hardware reciprocal of their product.
-__device__ __forceinline__ void rad_weights(int i, float a, RadW &lo, RadW &own) {
+// both radial weight pairs of a particle in cell i at offset a; t is the
+// first table entry of row i.  The lo family's denominators 2(l+1), 2(l+3)
+// (l = its staggered index, i or i-1) and the own family's 2i, 2(i+2) are all
+// entries of that row.
+__device__ __forceinline__ void rad_weights(int i, float a, const float4 &t, RadW &lo, RadW &own) {
   int ir = i;
   float al = a - 0.5f;
+  bool down = false;
   if (al <= 0.f) {
     ir = i - 1;
     al = a + 0.5f;
+    down = true;
   }
   if (ir < 0) {  // clamped at the axis as the reference does
     ir = 0;
     al = (float)i + a - 0.5f;
+    down = false;
   }
   const float fl = (float)ir, fi = (float)i;
-  const float d1 = 2.f * (fl + 1.f), d3 = 2.f * (fl + 3.f);
-  const float e0 = i == 0 ? 1.f : 2.f * fi, e2 = 2.f * (fi + 2.f);
-  const float p13 = d1 * d3, p02 = e0 * e2;
-  const float inv = __builtin_amdgcn_rcpf(p13 * p02);
-  const float i13 = p02 * inv, i02 = p13 * inv;  // 1 / (d1 d3), 1 / (e0 e2)
   lo.ir = ir;
-  lo.A = (1.f - al) * (2.f * fl + 1.f + al) * (d3 * i13);
-  lo.B = al * (2.f * fl + 2.f + al) * (d1 * i13);
+  lo.A = (1.f - al) * (2.f * fl + 1.f + al) * (down ? t.x : t.y);
+  lo.B = al * (2.f * fl + 2.f + al) * (down ? t.z : t.w);
   own.ir = i;
   if (i == 0) {
     double ad = a;  // axis cell in FP64: CYL_VOL annulus weight
     own.A = (float)((1.0 - ad) * ad * 4.0);
     own.B = (float)(ad * (1.0 + ad) / 4.0);
   } else {
-    own.A = (1.f - a) * (2.f * fi + a) * (e2 * i02);
-    own.B = a * (2.f * fi + 1.f + a) * (e0 * i02);
+    own.A = (1.f - a) * (2.f * fi + a) * t.x;
+    own.B = a * (2.f * fi + 1.f + a) * t.z;
   }
 }
 struct AxW {
@@ -119,7 +121,9 @@ __device__ __forceinline__ AxW ax_half(int k, float b) {
   return AxW{kz, be};
 }
 
-__device__ __forceinline__ float gather(const F32Args &p, const double *__restrict__ F, RadW rw, AxW aw) {
+// node values are rounded to FP32 and multiplied by the push's scale s
+// (escale for E, bscale for H) before the interpolation
+__device__ __forceinline__ float gather(const F32Args &p, const double *__restrict__ F, float s, RadW rw, AxW aw) {
   int ir = rw.ir, kz = aw.kz;
   if (ir > p.nr - 2 || kz > p.nz - 2) {
     atomicOr(p.err, kErrDomain);
@@ -128,7 +132,7 @@ __device__ __forceinline__ float gather(const F32Args &p, const double *__restri
   }
   const double *row0 = F + (size_t)ir * (size_t)p.nz + kz;
   const double *row1 = row0 + p.nz;
-  float f00 = (float)row0[0], f01 = (float)row0[1], f10 = (float)row1[0], f11 = (float)row1[1];
+  float f00 = (float)row0[0] * s, f01 = (float)row0[1] * s, f10 = (float)row1[0] * s, f11 = (float)row1[1] * s;
   float b = aw.b, ob = 1.f - aw.b;
   return (f00 * ob + f01 * b) * rw.A + (f10 * ob + f11 * b) * rw.B;
 }
@@ -140,14 +144,21 @@ __device__ __forceinline__ float gather(const F32Args &p, const double *__restri
 // FP32 expression on the same FP32-rounded node values, so the result does not
 // depend on the source.
 struct GlobalFields {
+  // the two table entries of row i (clamped only for memory safety: a corrupt
+  // cell word is caught by the gathers' domain check)
+  __device__ __forceinline__ void row(const F32Args &p, int i, float4 &t0, float4 &t1) const {
+    const int r = min(max(i, 0), p.nr - 1);
+    t0 = p.rows[2 * r];
+    t1 = p.rows[2 * r + 1];
+  }
   __device__ __forceinline__ void get(const F32Args &p, int, int, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
                                       float h[3]) const {
-    e[0] = gather(p, p.er, rl, zo);
-    e[1] = gather(p, p.ephi, ro, zo);
-    e[2] = gather(p, p.ez, ro, zh);
-    h[0] = gather(p, p.hhr, ro, zh);
-    h[1] = gather(p, p.hhphi, rl, zh);
-    h[2] = gather(p, p.hhz, rl, zo);
+    e[0] = gather(p, p.er, p.escale, rl, zo);
+    e[1] = gather(p, p.ephi, p.escale, ro, zo);
+    e[2] = gather(p, p.ez, p.escale, ro, zh);
+    h[0] = gather(p, p.hhr, p.bscale, ro, zh);
+    h[1] = gather(p, p.hhphi, p.bscale, rl, zh);
+    h[2] = gather(p, p.hhz, p.bscale, rl, zo);
   }
 };
 
@@ -156,6 +167,8 @@ struct GlobalFields {
 // (i, k) reads rows i-1..i+1 and columns k-1..k+1, so every cell with
 // i in [ai, ai+1] and k in [ak, ak+kFK-3] is served from the box
 constexpr int kFR = 4, kFK = 64, kFComp = 6, kFSize = kFComp * kFR * kFK;
+// the LDS array: the field box, then two float4 row entries for each box row
+constexpr int kFBoxFloats = kFSize + 8 * kFR;
 
 __device__ __forceinline__ float gather_box(const float *F, RadW rw, AxW aw, int bi, int bk) {
   const float *row0 = F + (rw.ir - bi) * kFK + (aw.kz - bk);
@@ -165,8 +178,18 @@ __device__ __forceinline__ float gather_box(const float *F, RadW rw, AxW aw, int
 }
 
 struct BoxFields {
-  const float *box;  // LDS, kFComp x kFR x kFK
+  const float *box;  // LDS, kFComp x kFR x kFK, then the row entries of rows ai-1 .. ai+2
   int ai, ak;        // first cell of the block
+  __device__ __forceinline__ void row(const F32Args &p, int i, float4 &t0, float4 &t1) const {
+    const int li = i - ai + 1;
+    if ((unsigned)li < (unsigned)kFR) {
+      const float4 *t = reinterpret_cast<const float4 *>(box + kFSize);
+      t0 = t[2 * li];
+      t1 = t[2 * li + 1];
+    } else {
+      GlobalFields().row(p, i, t0, t1);
+    }
+  }
   __device__ __forceinline__ void get(const F32Args &p, int i, int k, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
                                       float h[3]) const {
     const bool inbox =
@@ -183,12 +206,12 @@ struct BoxFields {
       e[0] = e0; e[1] = e1; e[2] = e2;
       h[0] = h0; h[1] = h1; h[2] = h2;
     } else {
-      const float e0 = gather(p, p.er, rl, zo);
-      const float e1 = gather(p, p.ephi, ro, zo);
-      const float e2 = gather(p, p.ez, ro, zh);
-      const float h0 = gather(p, p.hhr, ro, zh);
-      const float h1 = gather(p, p.hhphi, rl, zh);
-      const float h2 = gather(p, p.hhz, rl, zo);
+      const float e0 = gather(p, p.er, p.escale, rl, zo);
+      const float e1 = gather(p, p.ephi, p.escale, ro, zo);
+      const float e2 = gather(p, p.ez, p.escale, ro, zh);
+      const float h0 = gather(p, p.hhr, p.bscale, ro, zh);
+      const float h1 = gather(p, p.hhphi, p.bscale, rl, zh);
+      const float h2 = gather(p, p.hhz, p.bscale, rl, zo);
       e[0] = e0; e[1] = e1; e[2] = e2;
       h[0] = h0; h[1] = h1; h[2] = h2;
     }
@@ -199,15 +222,20 @@ struct BoxFields {
 // particle that the box serves reads them)
 __device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int ai, int ak) {
   const int bi = ai - 1, bk = ak - 1;
+  if (threadIdx.x < 2 * kFR) {
+    const int r = min(max(bi + (int)(threadIdx.x >> 1), 0), p.nr - 1);
+    reinterpret_cast<float4 *>(box + kFSize)[threadIdx.x] = p.rows[2 * r + (threadIdx.x & 1)];
+  }
   const double *src[kFComp] = {p.er, p.ephi, p.ez, p.hhr, p.hhphi, p.hhz};
 #pragma unroll
   for (int comp = 0; comp < kFComp; comp++) {
     const double *F = src[comp];
+    const float sc = comp < 3 ? p.escale : p.bscale;
     for (int t = threadIdx.x; t < kFR * kFK; t += blockDim.x) {
       const int r = t / kFK, col = t - r * kFK;
       const int gi = bi + r, gk = bk + col;
       float v = 0.f;
-      if ((unsigned)gi < (unsigned)p.nr && (unsigned)gk < (unsigned)p.nz) v = (float)F[(size_t)gi * p.nz + gk];
+      if ((unsigned)gi < (unsigned)p.nr && (unsigned)gk < (unsigned)p.nz) v = (float)F[(size_t)gi * p.nz + gk] * sc;
       box[comp * kFR * kFK + t] = v;
     }
   }
@@ -215,10 +243,11 @@ __device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int
 
 __device__ __forceinline__ float dot3f(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-// relativistic Boris on u = gamma v, src/particles.cpp:1025-1150
+// relativistic Boris on u = gamma v, src/particles.cpp:1025-1150; e and h
+// arrive multiplied by escale and bscale (the field sources apply them per node)
 __device__ __forceinline__ void boris_u(const F32Args &p, const float e[3], const float h[3], float u[3]) {
-  float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
-  float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
+  const float ee[3] = {e[0], e[1], e[2]};
+  float b[3] = {h[0], h[1], h[2]};
   for (int j = 0; j < 3; j++) u[j] += ee[j];
   float g = frsq(1.f + dot3f(u, u) * p.inv_c2);
   for (int j = 0; j < 3; j++) b[j] *= g;
@@ -230,10 +259,11 @@ __device__ __forceinline__ void boris_u(const F32Args &p, const float e[3], cons
   for (int j = 0; j < 3; j++) u[j] += ee[j];
 }
 
-// Vay on u, src/particles.cpp:1152-1254 (with the reference's b scaling)
+// Vay on u, src/particles.cpp:1152-1254 (with the reference's b scaling; e
+// and h arrive multiplied by escale and bscale)
 __device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const float h[3], float u[3]) {
-  float ee[3] = {e[0] * p.escale, e[1] * p.escale, e[2] * p.escale};
-  float b[3] = {h[0] * p.bscale, h[1] * p.bscale, h[2] * p.bscale};
+  const float ee[3] = {e[0], e[1], e[2]};
+  float b[3] = {h[0], h[1], h[2]};
   float t[3] = {u[0], u[1], u[2]};
   float g = frsq(1.f + dot3f(t, t) * p.inv_c2);
   float up[3];
@@ -343,7 +373,8 @@ __device__ __forceinline__ float m_of(float x) {
 // as differences of O(i) terms); they also reduce to strict_motion_weighting
 // (:786-988) for purely axial or radial motion.
 template <class S>
-__device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a, float b, float drho, float dzeta) {
+__device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a, float b, float drho, float dzeta,
+                        const float4 &t) {
   float jz0, jz1, jr0, jr1;
   if (i == 0) {
     double rs = a;  // axis cell in FP64
@@ -352,14 +383,13 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
     jr0 = drho * (0.5f * (1.f - b) - 0.25f * dzeta) * 2.f;
     jr1 = drho * (0.5f * b + 0.25f * dzeta) * 2.f;
   } else {
-    const float fi = (float)i, fi1 = fi + 1.f, fh = fi + 0.5f;
+    // t: second table entry of row i (1/i, 1/(i+1), 1/(i+1/2), i(i+1))
+    const float fi = (float)i;
     const float rs = fi + a, re = rs + drho;
-    // 1/rs, 1/re from one reciprocal; 1/i, 1/(i+1), 1/(i+1/2) from another
+    // 1/rs, 1/re from one reciprocal
     const float q = __builtin_amdgcn_rcpf(rs * re);
     const float irs = re * q, ire = rs * q;
-    const float ii = fi * fi1;
-    const float qi = __builtin_amdgcn_rcpf(ii * fh);
-    const float inv_i = fi1 * fh * qi, inv_i1 = fi * fh * qi, inv = ii * qi;
+    const float ii = t.w, inv_i = t.x, inv_i1 = t.y, inv = t.z;
     float x = drho * irs;
     float M = m_of(x);
     float N = -drho * ire - M;  // x / (1 + x) = drho / re
@@ -380,7 +410,7 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
 // deposited in its own cell
 template <class S>
 __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, int ko, float bo, int in_, float an,
-                          int kn, float bn) {
+                          int kn, float bn, float4 rt) {
   int di = in_ - io, dk = kn - ko;
   if (di < -1 || di > 1 || dk < -1 || dk > 1) {
     atomicOr(p.err, kErrDomain);
@@ -403,14 +433,15 @@ __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, in
     float sd_r = (t - tprev) * drho, sd_z = (t - tprev) * dzeta;
     if (rfirst) {
       float fr = di > 0 ? 1.f : 0.f;
-      segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z);
+      segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z, rt);
       ci += di;
+      rt = p.rows[2 * min(max(ci, 0), p.nr - 1) + 1];
       ca = di > 0 ? 0.f : 1.f;
       cb += sd_z;
       tr = 2.f;
     } else {
       float fz = dk > 0 ? 1.f : 0.f;
-      segment(p, sk, w, ci, ck, ca, cb, sd_r, fz - cb);
+      segment(p, sk, w, ci, ck, ca, cb, sd_r, fz - cb, rt);
       ck += dk;
       cb = dk > 0 ? 0.f : 1.f;
       ca += sd_r;
@@ -419,7 +450,7 @@ __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, in
     tprev = t;
   }
   // last piece to the new position
-  segment(p, sk, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb);
+  segment(p, sk, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb, rt);
 }
 
 // wall handling of move_half_reflect on the Cartesian half-step point
@@ -489,13 +520,15 @@ __device__ __forceinline__ const T &at(const T *p, unsigned off) {
 
 __device__ __forceinline__ PState load_state(const F32Args &p, unsigned off) {
   PState s;
-  s.c = at(p.cell, off);
-  s.a = at(p.fa, off);
-  s.b = at(p.fb, off);
-  s.u0 = at(p.ur, off);
-  s.u1 = at(p.uphi, off);
-  s.u2 = at(p.uz, off);
-  s.w = at(p.w, off);
+  // streaming (non-temporal) loads and stores: each slot is touched once per
+  // step, so the state should not displace the fields and tables from the caches
+  s.c = __builtin_nontemporal_load(&at(p.cell, off));
+  s.a = __builtin_nontemporal_load(&at(p.fa, off));
+  s.b = __builtin_nontemporal_load(&at(p.fb, off));
+  s.u0 = __builtin_nontemporal_load(&at(p.ur, off));
+  s.u1 = __builtin_nontemporal_load(&at(p.uphi, off));
+  s.u2 = __builtin_nontemporal_load(&at(p.uz, off));
+  s.w = __builtin_nontemporal_load(&at(p.w, off));
   // all seven loads are issued together, ahead of the dead-particle branch
   // (instead of the six that the branch guards waiting for the cell word)
   asm volatile("" ::"v"(s.c), "v"(s.a), "v"(s.b), "v"(s.u0), "v"(s.u1), "v"(s.u2), "v"(s.w));
@@ -514,7 +547,9 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
   sk.begin_cell(i, k);
   // gathers (E at the full step, H at the half-time copy)
   RadW rl, ro;
-  rad_weights(i, a, rl, ro);
+  float4 t0, t1;
+  fs.row(p, i, t0, t1);
+  rad_weights(i, a, t0, rl, ro);
   AxW zo = ax_own(k, b), zh = ax_half(k, b);
   float e[3], h[3];
   fs.get(p, i, k, rl, ro, zo, zh, e, h);
@@ -540,7 +575,7 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
   renorm(i1, xa2);
   renorm(k1, b2);
   if (alive) {
-    deposit_j(p, sk, w, i, a, k, b, i1, xa2, k1, b2);
+    deposit_j(p, sk, w, i, a, k, b, i1, xa2, k1, b2, t1);
     // back_velocity_to_rz with the second half step's angle (:1005-1023)
     float u0 = ct * u[0] - st * u[1];
     float u1 = st * u[0] + ct * u[1];
@@ -552,12 +587,12 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
   // too, instead of reusing 64-bit addresses formed for the loads
   unsigned so = off;
   asm("" : "+v"(so));
-  at(p.cell, so) = pack(i1, k1) | (alive ? 0u : kDead);
-  at(p.fa, so) = xa2;
-  at(p.fb, so) = b2;
-  at(p.ur, so) = u[0];
-  at(p.uphi, so) = u[1];
-  at(p.uz, so) = u[2];
+  __builtin_nontemporal_store(pack(i1, k1) | (alive ? 0u : kDead), &at(p.cell, so));
+  __builtin_nontemporal_store(xa2, &at(p.fa, so));
+  __builtin_nontemporal_store(b2, &at(p.fb, so));
+  __builtin_nontemporal_store(u[0], &at(p.ur, so));
+  __builtin_nontemporal_store(u[1], &at(p.uphi, so));
+  __builtin_nontemporal_store(u[2], &at(p.uz, so));
 }
 
 // the whole step of the particle in slot `slot` of the arrays of p (which
@@ -623,7 +658,7 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
 template <int PPT, int WAVES>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_fused32_tiled(F32Args p, GlobalSink gs, long long n_cap) {
   __shared__ double box[TileSink::kSize];
-  __shared__ float fbox[kFSize];
+  __shared__ __align__(16) float fbox[kFBoxFloats];
   constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
   fused32_tile<PPT, WAVES>(p, gs, (long long)blockIdx.x * kTile, n_cap, box, fbox);
 }
@@ -685,7 +720,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                                                      const GlobalSink *__restrict__ sinks,
                                                      const EnsItem *__restrict__ items, int n_items) {
   __shared__ double box[TileSink::kSize];
-  __shared__ float fbox[kFSize];
+  __shared__ __align__(16) float fbox[kFBoxFloats];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const EnsItem item = items[it];
     const F32Args p = args[item.arg];
@@ -819,6 +854,7 @@ F32Args args32(Ctx &c, Species &sp) {
   p.hhr = c.a[kHHR];
   p.hhphi = c.a[kHHPhi];
   p.hhz = c.a[kHHZ];
+  p.rows = c.rows;
   p.err = c.err;
   p.cell = (uint32_t *)sp.f32.cell;
   p.fa = sp.f32.fr;
