@@ -1,12 +1,14 @@
-// Stable counting sort of a species' particles by cell (no reference
-// counterpart: it keeps gathers and deposits of neighbouring particles on the
-// same cache lines).  Key = i * (nz-1) + k of the reference's cell index
-// (include/geometry.h:44, clamped at 0); dead particles get the next key and
-// end up behind the living ones, chunk padding slots after those.  The order
-// among equal keys is the input order, so the result equals a stable sort by
-// key; it is written back in the chunk-transposed layout.
+// Sort of a species' particles by cell (no reference counterpart: it keeps
+// gathers and deposits of neighbouring particles on the same cache lines).
+// Key = i * (nz-1) + k of the reference's cell index (include/geometry.h:44,
+// clamped at 0); dead particles get the next key and end up behind the living
+// ones, chunk padding slots after those.  The result is written back in the
+// chunk-transposed layout.
 //
-// LSD radix sort with 8-bit digits (one counting sort per digit):
+// Grids of up to 2^26 cells: one counting sort over the whole key, with one
+// atomic per run of equal keys in a wavefront (the order among equal keys
+// from different wavefronts is not kept).  Larger grids: a stable LSD radix
+// sort with 8-bit digits (one counting sort per digit):
 //   hist:    per block of 16384 elements, a 256-bin digit histogram
 //   scan:    exclusive scan of the bin-major histogram matrix
 //   scatter: per block, stable ranks from wavefront ballots (lanes holding
@@ -127,6 +129,113 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t *__restrict
   }
 }
 
+// ---- counting sort by the whole key (grids of up to kCountBins cells) ----
+//
+// The particles of a sorted species stay nearly sorted between sorts, so the
+// 64 consecutive keys of a wavefront form a few runs of equal keys.  Each run
+// is counted (pass 1) and placed (pass 2) with one global atomic by its first
+// lane; the lanes of a run take consecutive positions in their input order.
+// Runs of equal keys from different wavefronts are placed in the order their
+// atomics arrive, so the result is sorted by key but not stable across
+// wavefronts.
+constexpr uint32_t kCountBins = 1u << 26;
+
+__global__ void k_fill_u32(uint32_t *p, long long n, uint32_t v) {
+  long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) p[j] = v;
+}
+
+// this lane's run: first lane, length, and whether this lane leads it
+__device__ __forceinline__ void key_run(uint32_t key, bool valid, int &head, int &len) {
+  const unsigned lane = __lane_id();
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool starts = valid && (lane == 0 || prev != key);
+  const uint64_t heads = __ballot(starts);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const uint64_t hm = heads & upto;                     // heads at or below this lane
+  head = hm ? 63 - __clzll(hm) : 0;
+  const uint64_t after = heads & ~upto;                 // heads above this lane
+  const int nvalid = __popcll(__ballot(valid));         // the valid lanes are a prefix
+  const int end = after ? __ffsll((long long)after) - 1 : nvalid;
+  len = end - head;
+}
+
+// sort keys straight from the particle state, by logical index s (the state
+// is read through the chunk-transposed layout)
+struct KeysF32 {
+  GridDesc g;
+  const uint32_t *cell;
+  uint32_t dead_key;
+  __device__ __forceinline__ uint32_t operator()(long long s) const {
+    const uint32_t c = cell[phys_of(s)];
+    if (c == kPadCell) return dead_key + 1;
+    if (c & 0x80000000u) return dead_key;
+    return ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
+  }
+};
+struct KeysF64 {
+  GridDesc g;
+  const double *r, *z;
+  const uint8_t *alive;
+  uint32_t dead_key;
+  __device__ __forceinline__ uint32_t operator()(long long s) const {
+    const long long j = phys_of(s);
+    if (alive[j] == kPadSlot) return dead_key + 1;
+    if (alive[j] != 1) return dead_key;
+    int i = (int)ceil(r[j] / g.dr) - 1, k = (int)ceil(z[j] / g.dz) - 1;
+    i = max(0, min(i, g.nr - 2));
+    k = max(0, min(k, g.nz - 2));
+    return (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
+  }
+};
+
+template <class K>
+__global__ void __launch_bounds__(256) k_count_runs(K keys, long long n, uint32_t *__restrict__ cnt) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = s < n;
+  const uint32_t key = valid ? keys(s) : 0xFFFFFFFFu;
+  int head, len;
+  key_run(key, valid, head, len);
+  if (valid && head == (int)__lane_id()) atomicAdd(&cnt[key], (uint32_t)len);
+}
+
+// cnt holds each key's first position (exclusive scan of the counts); it is
+// advanced as runs are placed.  src[slot of sorted rank] = slot of the particle
+template <class K>
+__global__ void __launch_bounds__(256) k_place_runs(K keys, long long n, uint32_t *__restrict__ cnt,
+                                                    uint32_t *__restrict__ src) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = s < n;
+  const uint32_t key = valid ? keys(s) : 0xFFFFFFFFu;
+  int head, len;
+  key_run(key, valid, head, len);
+  const unsigned lane = __lane_id();
+  uint32_t base = 0;
+  if (valid && head == (int)lane) base = atomicAdd(&cnt[key], (uint32_t)len);
+  base = __shfl(base, head);
+  if (valid) src[phys_of((long long)base + (lane - (unsigned)head))] = (uint32_t)phys_of(s);
+}
+
+// every particle array in one pass: out[p] = in[src[p]]
+struct Arrays7 {
+  void *a[7];
+};
+__global__ void k_gather_f32(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src, long long n) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t q = src[p];
+#pragma unroll
+  for (int k = 0; k < 7; k++) static_cast<uint32_t *>(out.a[k])[p] = static_cast<const uint32_t *>(in.a[k])[q];
+}
+__global__ void k_gather_f64(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src, long long n) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t q = src[p];
+#pragma unroll
+  for (int k = 0; k < 6; k++) static_cast<double *>(out.a[k])[p] = static_cast<const double *>(in.a[k])[q];
+  static_cast<uint8_t *>(out.a[6])[p] = static_cast<const uint8_t *>(in.a[6])[q];
+}
+
 // keys in logical order (so that ties keep their logical order); vals = slot
 __global__ void k_keys_f64(GridDesc g, const double *r, const double *z, const uint8_t *alive, long long n,
                            uint32_t *keys, uint32_t *vals, uint32_t dead_key) {
@@ -173,6 +282,16 @@ void exclusive_scan(uint32_t *data, long long n, uint32_t *tmp, hipStream_t s) {
 }
 
 template <typename T>
+bool ensure_alt(T *&alt, long long n) {
+  if (alt) return true;
+  if (hipMalloc((void **)&alt, (size_t)n * sizeof(T)) != hipSuccess) {
+    alt = nullptr;
+    return false;
+  }
+  return true;
+}
+
+template <typename T>
 bool gather_swap(T *&a, T *&alt, const uint32_t *perm, long long n, hipStream_t s) {
   if (!a) return true;
   if (!alt && hipMalloc((void **)&alt, (size_t)n * sizeof(T)) != hipSuccess) {
@@ -198,9 +317,13 @@ int sort_species(Ctx &c, Species &sp) {
   while ((1ull << bits) <= (unsigned long long)dead_key + 1) bits++;
   const int passes = (bits + 7) / 8;
   const int nblocks = (int)((n + kPerBlock - 1) / kPerBlock);
-  const long long hist_n = (long long)kBins * nblocks;
-  // workspace: 4 key/val buffers + histogram + scan scratch
-  size_t need = (size_t)n * 4 * sizeof(uint32_t) + (size_t)(hist_n + hist_n / kScanBlock + 4096) * 2 * sizeof(uint32_t);
+  const bool counting = (unsigned long long)dead_key + 2 <= kCountBins;
+  // counting sort: one bin per key; radix sort: the per-block digit histograms
+  const long long hist_n = counting ? (long long)dead_key + 2 : (long long)kBins * nblocks;
+  // workspace: key/val buffers (counting sort: the source slots only) +
+  // histogram + scan scratch
+  const size_t bufs = counting ? 1 : 4;
+  size_t need = (size_t)n * bufs * sizeof(uint32_t) + (size_t)(hist_n + hist_n / kScanBlock + 4096) * 2 * sizeof(uint32_t);
   if (c.sort_ws_bytes < need) {
     if (c.sort_ws) (void)hipFree(c.sort_ws);
     c.sort_ws = nullptr;
@@ -209,8 +332,57 @@ int sort_species(Ctx &c, Species &sp) {
     c.sort_ws_bytes = need;
   }
   uint32_t *ka = (uint32_t *)c.sort_ws, *va = ka + n, *kb = va + n, *vb = kb + n;
-  uint32_t *hist = vb + n, *tmp = hist + hist_n;
+  uint32_t *hist = (uint32_t *)c.sort_ws + bufs * n, *tmp = hist + hist_n;
   unsigned eb = (unsigned)((n + 255) / 256);
+  if (counting) {
+    // count runs of keys, scan, place runs: src[slot] = source slot; then all
+    // arrays in one gather
+    const unsigned hb = (unsigned)((hist_n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(hb), dim3(256), 0, c.stream, hist, hist_n, 0u);
+    Arrays7 in{}, out{};
+    unsigned gb = eb;
+    if (c.precision == 1) {
+      SpeciesF32 &a = sp.f32, &b = sp.f32_alt;
+      if (!a.cell) return 0;
+      if (!(ensure_alt(b.cell, n) && ensure_alt(b.fr, n) && ensure_alt(b.fz, n) && ensure_alt(b.vr, n) &&
+            ensure_alt(b.vphi, n) && ensure_alt(b.vz, n) && ensure_alt(b.w, n)))
+        return -3;
+      const KeysF32 keys{c.g, (const uint32_t *)a.cell, dead_key};
+      hipLaunchKernelGGL(k_count_runs<KeysF32>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist);
+      exclusive_scan(hist, hist_n, tmp, c.stream);
+      hipLaunchKernelGGL(k_place_runs<KeysF32>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist, ka);
+      in = Arrays7{{a.cell, a.fr, a.fz, a.vr, a.vphi, a.vz, a.w}};
+      out = Arrays7{{b.cell, b.fr, b.fz, b.vr, b.vphi, b.vz, b.w}};
+      hipLaunchKernelGGL(k_gather_f32, dim3(gb), dim3(256), 0, c.stream, in, out, ka, n);
+      std::swap(a.cell, b.cell);
+      std::swap(a.fr, b.fr);
+      std::swap(a.fz, b.fz);
+      std::swap(a.vr, b.vr);
+      std::swap(a.vphi, b.vphi);
+      std::swap(a.vz, b.vz);
+      std::swap(a.w, b.w);
+    } else {
+      SpeciesF64 &a = sp.f64, &b = sp.f64_alt;
+      if (!(ensure_alt(b.r, n) && ensure_alt(b.z, n) && ensure_alt(b.vr, n) && ensure_alt(b.vphi, n) &&
+            ensure_alt(b.vz, n) && ensure_alt(b.w, n) && ensure_alt(b.alive, n)))
+        return -3;
+      const KeysF64 keys{c.g, a.r, a.z, a.alive, dead_key};
+      hipLaunchKernelGGL(k_count_runs<KeysF64>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist);
+      exclusive_scan(hist, hist_n, tmp, c.stream);
+      hipLaunchKernelGGL(k_place_runs<KeysF64>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist, ka);
+      in = Arrays7{{a.r, a.z, a.vr, a.vphi, a.vz, a.w, a.alive}};
+      out = Arrays7{{b.r, b.z, b.vr, b.vphi, b.vz, b.w, b.alive}};
+      hipLaunchKernelGGL(k_gather_f64, dim3(gb), dim3(256), 0, c.stream, in, out, ka, n);
+      std::swap(a.r, b.r);
+      std::swap(a.z, b.z);
+      std::swap(a.vr, b.vr);
+      std::swap(a.vphi, b.vphi);
+      std::swap(a.vz, b.vz);
+      std::swap(a.w, b.w);
+      std::swap(a.alive, b.alive);
+    }
+    return 0;
+  }
   if (c.precision == 1) {
     if (!sp.f32.cell) return 0;
     hipLaunchKernelGGL(k_keys_f32, dim3(eb), dim3(256), 0, c.stream, c.g, (const uint32_t *)sp.f32.cell, n, ka, va,
