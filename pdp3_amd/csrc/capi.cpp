@@ -285,7 +285,7 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
   for (int k = 0; k < kNComp; k++) c.a[k] = c.grid + (size_t)k * n;
   if ((st = dalloc(ctx, &c.koef_e, n)) || (st = dalloc(ctx, &c.koef_h, n)) || (st = dalloc(ctx, &c.err, 1)))
     return bail(st);
-  if (hipHostMalloc((void **)&c.err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&c.err_host, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
     return bail(PDP3_ERR_HIP);
   *c.err_host = 0;
   if (hipMemsetAsync(c.grid, 0, n * kNComp * sizeof(double), c.stream) != hipSuccess) return bail(PDP3_ERR_HIP);

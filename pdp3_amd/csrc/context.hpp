@@ -126,7 +126,7 @@ struct Ctx {
   std::vector<int> order;  // ParticlesList order of created species
   int next_bunch = 0;
   int *err = nullptr;      // device error bits
-  int *err_host = nullptr; // pinned mirror
+  int *err_host = nullptr; // pinned: [0] mirror of err, [1] the sort's run count
   std::string last_error;
   bool profiling = false;
   KernelTimer timers[5];

@@ -1,20 +1,27 @@
-// Sort of a species' particles by cell (no reference counterpart: it keeps
-// gathers and deposits of neighbouring particles on the same cache lines).
-// Key = i * (nz-1) + k of the reference's cell index (include/geometry.h:44,
-// clamped at 0); dead particles get the next key and end up behind the living
-// ones, chunk padding slots after those.  The result is written back in the
+// Stable sort of a species' particles by cell (no reference counterpart: it
+// keeps gathers and deposits of neighbouring particles on the same cache
+// lines).  Key = i * (nz-1) + k of the reference's cell index
+// (include/geometry.h:44, clamped at 0); dead particles get the next key and
+// end up behind the living ones, chunk padding slots after those.  The order
+// among equal keys is the input order; the result is written back in the
 // chunk-transposed layout.
 //
-// Grids of up to 2^26 cells: one counting sort over the whole key, with one
-// atomic per run of equal keys in a wavefront (the order among equal keys
-// from different wavefronts is not kept).  Larger grids: a stable LSD radix
-// sort with 8-bit digits (one counting sort per digit):
-//   hist:    per block of 16384 elements, a 256-bin digit histogram
-//   scan:    exclusive scan of the bin-major histogram matrix
-//   scatter: per block, stable ranks from wavefront ballots (lanes holding
-//            the same digit are matched with 8 ballots), running per-wave
-//            digit counters in LDS, then a coalesced-per-digit write
-// then every particle array is gathered through the resulting permutation.
+// A sorted species stays nearly sorted between sorts, so the sort works on
+// runs rather than particles:
+//   runs:    each wavefront's 64 consecutive keys split into runs of equal
+//            keys (a run never spans two wavefronts); one record (key,
+//            length) per run, in logical order
+//   records: stable LSD radix sort of the records by key, 8-bit digits:
+//              hist:    per block of 16384 records, a 256-bin digit histogram
+//              scan:    exclusive scan of the bin-major histogram matrix
+//              scatter: per block, stable ranks from wavefront ballots (lanes
+//                       holding the same digit are matched with 8 ballots),
+//                       running per-wave digit counters in LDS
+//   ranks:   exclusive scan of the run lengths in sorted order: each run's
+//            first rank; a particle's rank = its run's + its place in the run
+// then every particle array is gathered through the resulting permutation in
+// one pass (for a particle that stayed in place the gather reads its own
+// slot, so a nearly sorted species is moved with coalesced reads).
 #include "context.hpp"
 
 namespace pdp3 {
@@ -129,91 +136,100 @@ __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t *__restrict
   }
 }
 
-// ---- counting sort by the whole key (grids of up to kCountBins cells) ----
-//
-// The particles of a sorted species stay nearly sorted between sorts, so the
-// 64 consecutive keys of a wavefront form a few runs of equal keys.  Each run
-// is counted (pass 1) and placed (pass 2) with one global atomic by its first
-// lane; the lanes of a run take consecutive positions in their input order.
-// Runs of equal keys from different wavefronts are placed in the order their
-// atomics arrive, so the result is sorted by key but not stable across
-// wavefronts.
-constexpr uint32_t kCountBins = 1u << 26;
+// ---- runs of equal keys ----
 
 __global__ void k_fill_u32(uint32_t *p, long long n, uint32_t v) {
   long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) p[j] = v;
 }
 
-// this lane's run: first lane, length, and whether this lane leads it
-__device__ __forceinline__ void key_run(uint32_t key, bool valid, int &head, int &len) {
+// the runs of this wavefront's keys (lanes >= the valid prefix are not
+// keys): heads = mask of first lanes; head, len = this lane's run
+__device__ __forceinline__ uint64_t wave_runs(uint32_t key, bool valid, int &head, int &len) {
   const unsigned lane = __lane_id();
   const uint32_t prev = __shfl_up(key, 1);
   const bool starts = valid && (lane == 0 || prev != key);
   const uint64_t heads = __ballot(starts);
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  const uint64_t hm = heads & upto;                     // heads at or below this lane
+  const uint64_t hm = heads & upto;  // heads at or below this lane
   head = hm ? 63 - __clzll(hm) : 0;
-  const uint64_t after = heads & ~upto;                 // heads above this lane
-  const int nvalid = __popcll(__ballot(valid));         // the valid lanes are a prefix
+  const uint64_t after = heads & ~upto;
+  const int nvalid = __popcll(__ballot(valid));  // the valid lanes are a prefix
   const int end = after ? __ffsll((long long)after) - 1 : nvalid;
   len = end - head;
+  return heads;
 }
 
-// sort keys straight from the particle state, by logical index s (the state
-// is read through the chunk-transposed layout)
-struct KeysF32 {
-  GridDesc g;
-  const uint32_t *cell;
-  uint32_t dead_key;
-  __device__ __forceinline__ uint32_t operator()(long long s) const {
-    const uint32_t c = cell[phys_of(s)];
-    if (c == kPadCell) return dead_key + 1;
-    if (c & 0x80000000u) return dead_key;
-    return ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
-  }
-};
-struct KeysF64 {
-  GridDesc g;
-  const double *r, *z;
-  const uint8_t *alive;
-  uint32_t dead_key;
-  __device__ __forceinline__ uint32_t operator()(long long s) const {
-    const long long j = phys_of(s);
-    if (alive[j] == kPadSlot) return dead_key + 1;
-    if (alive[j] != 1) return dead_key;
-    int i = (int)ceil(r[j] / g.dr) - 1, k = (int)ceil(z[j] / g.dz) - 1;
-    i = max(0, min(i, g.nr - 2));
-    k = max(0, min(k, g.nz - 2));
-    return (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
-  }
-};
-
+// runs per block of 256 keys (in logical order: logical index s is in slot
+// phys_of(s))
 template <class K>
-__global__ void __launch_bounds__(256) k_count_runs(K keys, long long n, uint32_t *__restrict__ cnt) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_run_count(K keys, long long n, uint32_t *__restrict__ block_runs) {
+  __shared__ uint32_t wc[4];
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   const bool valid = s < n;
-  const uint32_t key = valid ? keys(s) : 0xFFFFFFFFu;
   int head, len;
-  key_run(key, valid, head, len);
-  if (valid && head == (int)__lane_id()) atomicAdd(&cnt[key], (uint32_t)len);
+  const uint64_t heads = wave_runs(valid ? keys(phys_of(s)) : 0u, valid, head, len);
+  if (__lane_id() == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(heads);
+  __syncthreads();
+  if (threadIdx.x == 0) block_runs[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
-// cnt holds each key's first position (exclusive scan of the counts); it is
-// advanced as runs are placed.  src[slot of sorted rank] = slot of the particle
+// the record index of this lane's run (block_base: exclusive scan of the
+// block counts)
+__device__ __forceinline__ uint32_t run_index(uint64_t heads, int head, const uint32_t *block_base) {
+  __shared__ uint32_t wc[4];
+  const int wave = threadIdx.x >> 6;
+  if (__lane_id() == 0) wc[wave] = (uint32_t)__popcll(heads);
+  __syncthreads();
+  uint32_t r = block_base[blockIdx.x];
+  for (int w = 0; w < wave; w++) r += wc[w];
+  const uint64_t below = head == 0 ? 0ull : (heads & (~0ull >> (64 - head)));
+  return r + (uint32_t)__popcll(below);
+}
+
 template <class K>
-__global__ void __launch_bounds__(256) k_place_runs(K keys, long long n, uint32_t *__restrict__ cnt,
-                                                    uint32_t *__restrict__ src) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_run_emit(K keys, long long n, const uint32_t *__restrict__ block_base,
+                                                  uint32_t *__restrict__ rkey, uint32_t *__restrict__ ridx,
+                                                  uint32_t *__restrict__ rlen) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   const bool valid = s < n;
-  const uint32_t key = valid ? keys(s) : 0xFFFFFFFFu;
+  const uint32_t key = valid ? keys(phys_of(s)) : 0u;
   int head, len;
-  key_run(key, valid, head, len);
+  const uint64_t heads = wave_runs(key, valid, head, len);
+  const uint32_t r = run_index(heads, head, block_base);
+  if (valid && head == (int)__lane_id()) {
+    rkey[r] = key;
+    ridx[r] = r;
+    rlen[r] = (uint32_t)len;
+  }
+}
+
+// lens[t] = length of the t-th run in key order; after the scan of lens,
+// rfirst[run] = its first rank
+__global__ void k_run_lens(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rlen, long long m,
+                           uint32_t *__restrict__ lens) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) lens[t] = rlen[order[t]];
+}
+__global__ void k_run_first(const uint32_t *__restrict__ order, const uint32_t *__restrict__ lens, long long m,
+                            uint32_t *__restrict__ rfirst) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) rfirst[order[t]] = lens[t];
+}
+
+// src[slot of the particle's rank] = its slot
+template <class K>
+__global__ void __launch_bounds__(256) k_run_place(K keys, long long n, const uint32_t *__restrict__ block_base,
+                                                   const uint32_t *__restrict__ rfirst, uint32_t *__restrict__ src) {
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = s < n;
+  int head, len;
+  const uint64_t heads = wave_runs(valid ? keys(phys_of(s)) : 0u, valid, head, len);
+  const uint32_t r = run_index(heads, head, block_base);
   const unsigned lane = __lane_id();
-  uint32_t base = 0;
-  if (valid && head == (int)lane) base = atomicAdd(&cnt[key], (uint32_t)len);
-  base = __shfl(base, head);
-  if (valid) src[phys_of((long long)base + (lane - (unsigned)head))] = (uint32_t)phys_of(s);
+  const uint32_t first = valid && head == (int)lane ? rfirst[r] : 0u;
+  const uint32_t rank = __shfl(first, head) + (lane - (unsigned)head);
+  if (valid) src[phys_of((long long)rank)] = (uint32_t)phys_of(s);
 }
 
 // every particle array in one pass: out[p] = in[src[p]]
@@ -236,41 +252,32 @@ __global__ void k_gather_f64(Arrays7 in, Arrays7 out, const uint32_t *__restrict
   static_cast<uint8_t *>(out.a[6])[p] = static_cast<const uint8_t *>(in.a[6])[q];
 }
 
-// keys in logical order (so that ties keep their logical order); vals = slot
-__global__ void k_keys_f64(GridDesc g, const double *r, const double *z, const uint8_t *alive, long long n,
-                           uint32_t *keys, uint32_t *vals, uint32_t dead_key) {
-  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  long long j = phys_of(s);
-  uint32_t key = alive[j] == kPadSlot ? dead_key + 1 : dead_key;
-  if (alive[j] == 1) {
+// the sort key of the particle in slot j, from its state
+struct KeysF32 {
+  GridDesc g;
+  const uint32_t *cell;
+  uint32_t dead_key;
+  __device__ __forceinline__ uint32_t operator()(long long j) const {
+    const uint32_t c = cell[j];
+    if (c == kPadCell) return dead_key + 1;
+    if (c & 0x80000000u) return dead_key;
+    return ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
+  }
+};
+struct KeysF64 {
+  GridDesc g;
+  const double *r, *z;
+  const uint8_t *alive;
+  uint32_t dead_key;
+  __device__ __forceinline__ uint32_t operator()(long long j) const {
+    if (alive[j] == kPadSlot) return dead_key + 1;
+    if (alive[j] != 1) return dead_key;
     int i = (int)ceil(r[j] / g.dr) - 1, k = (int)ceil(z[j] / g.dz) - 1;
     i = max(0, min(i, g.nr - 2));
     k = max(0, min(k, g.nz - 2));
-    key = (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
+    return (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
   }
-  keys[s] = key;
-  vals[s] = (uint32_t)j;
-}
-
-__global__ void k_keys_f32(GridDesc g, const uint32_t *cell, long long n, uint32_t *keys, uint32_t *vals,
-                           uint32_t dead_key) {
-  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  long long j = phys_of(s);
-  uint32_t c = cell[j];
-  uint32_t key = c == kPadCell ? dead_key + 1 : dead_key;
-  if (!(c & 0x80000000u)) key = ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
-  keys[s] = key;
-  vals[s] = (uint32_t)j;
-}
-
-// slot p of the chunked layout holds sorted rank logical_of(p)
-template <typename T>
-__global__ void k_gather(const T *__restrict__ in, T *__restrict__ out, const uint32_t *__restrict__ perm, long long n) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) out[p] = in[perm[logical_of(p)]];
-}
+};
 
 void exclusive_scan(uint32_t *data, long long n, uint32_t *tmp, hipStream_t s) {
   long long nb = (n + kScanBlock - 1) / kScanBlock;
@@ -280,29 +287,39 @@ void exclusive_scan(uint32_t *data, long long n, uint32_t *tmp, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanBlock), 0, s, data, n, tmp);
   }
 }
-
-template <typename T>
-bool ensure_alt(T *&alt, long long n) {
-  if (alt) return true;
-  if (hipMalloc((void **)&alt, (size_t)n * sizeof(T)) != hipSuccess) {
-    alt = nullptr;
-    return false;
+// scratch words exclusive_scan needs for n elements
+long long scan_scratch(long long n) {
+  long long t = 0;
+  while (n > 1) {
+    n = (n + kScanBlock - 1) / kScanBlock;
+    t += n;
   }
-  return true;
+  return t + 1;
 }
 
-template <typename T>
-bool gather_swap(T *&a, T *&alt, const uint32_t *perm, long long n, hipStream_t s) {
-  if (!a) return true;
-  if (!alt && hipMalloc((void **)&alt, (size_t)n * sizeof(T)) != hipSuccess) {
+// stable LSD radix sort of m (key, val) pairs by the low `bits` bits of the
+// key; returns the buffers holding the result (a or b)
+void radix_pairs(uint32_t *&ka, uint32_t *&va, uint32_t *&kb, uint32_t *&vb, long long m, int bits, uint32_t *hist,
+                 uint32_t *tmp, hipStream_t st) {
+  const int passes = (bits + 7) / 8;
+  const int nblocks = (int)((m + kPerBlock - 1) / kPerBlock);
+  const long long hist_n = (long long)kBins * nblocks;
+  for (int p = 0; p < passes; p++) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(kThreads), 0, st, ka, m, shift, hist, nblocks);
+    exclusive_scan(hist, hist_n, tmp, st);
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kThreads), 0, st, ka, va, kb, vb, m, shift, hist, nblocks);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+}
+
+bool ensure_alt(void *&alt, size_t bytes) {
+  if (alt) return true;
+  if (hipMalloc(&alt, bytes) != hipSuccess) {
     alt = nullptr;
     return false;
   }
-  unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, s, a, alt, perm, n);
-  T *t = a;
-  a = alt;
-  alt = t;
   return true;
 }
 
@@ -311,119 +328,116 @@ bool gather_swap(T *&a, T *&alt, const uint32_t *perm, long long n, hipStream_t 
 int sort_species(Ctx &c, Species &sp) {
   const long long n = sp.n_cap;
   if (n <= 1 || !sp.chunked) return 0;
+  if (c.precision == 1 && !sp.f32.cell) return 0;
   const uint32_t ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
   const uint32_t dead_key = ncell;
   int bits = 1;
   while ((1ull << bits) <= (unsigned long long)dead_key + 1) bits++;
-  const int passes = (bits + 7) / 8;
-  const int nblocks = (int)((n + kPerBlock - 1) / kPerBlock);
-  const bool counting = (unsigned long long)dead_key + 2 <= kCountBins;
-  // counting sort: one bin per key; radix sort: the per-block digit histograms
-  const long long hist_n = counting ? (long long)dead_key + 2 : (long long)kBins * nblocks;
-  // workspace: key/val buffers (counting sort: the source slots only) +
-  // histogram + scan scratch
-  const size_t bufs = counting ? 1 : 4;
-  size_t need = (size_t)n * bufs * sizeof(uint32_t) + (size_t)(hist_n + hist_n / kScanBlock + 4096) * 2 * sizeof(uint32_t);
-  if (c.sort_ws_bytes < need) {
+  const long long nb = (n + 255) / 256;  // blocks of the run kernels
+  // workspace (u32 words): src (n); block counts (nb + 1); scan scratch;
+  // then the records (5 per run, sized after counting them)
+  // scratch: the scans of the block counts and of the run lengths, and the
+  // records' digit histograms with their scan (at most n records)
+  const long long rec_hist = (long long)kBins * ((n + kPerBlock - 1) / kPerBlock);
+  const long long scratch =
+      std::max(std::max(scan_scratch(nb + 1), scan_scratch(n)), rec_hist + scan_scratch(rec_hist)) + 1024;
+  const long long fixed = n + (nb + 1) + scratch;
+  auto ensure_ws = [&](long long words) -> bool {
+    const size_t need = (size_t)words * sizeof(uint32_t);
+    if (c.sort_ws_bytes >= need) return true;
     if (c.sort_ws) (void)hipFree(c.sort_ws);
     c.sort_ws = nullptr;
     c.sort_ws_bytes = 0;
-    if (hipMalloc(&c.sort_ws, need) != hipSuccess) return -3;
+    if (hipMalloc(&c.sort_ws, need) != hipSuccess) return false;
     c.sort_ws_bytes = need;
-  }
-  uint32_t *ka = (uint32_t *)c.sort_ws, *va = ka + n, *kb = va + n, *vb = kb + n;
-  uint32_t *hist = (uint32_t *)c.sort_ws + bufs * n, *tmp = hist + hist_n;
-  unsigned eb = (unsigned)((n + 255) / 256);
-  if (counting) {
-    // count runs of keys, scan, place runs: src[slot] = source slot; then all
-    // arrays in one gather
-    const unsigned hb = (unsigned)((hist_n + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u32, dim3(hb), dim3(256), 0, c.stream, hist, hist_n, 0u);
-    Arrays7 in{}, out{};
-    unsigned gb = eb;
-    if (c.precision == 1) {
-      SpeciesF32 &a = sp.f32, &b = sp.f32_alt;
-      if (!a.cell) return 0;
-      if (!(ensure_alt(b.cell, n) && ensure_alt(b.fr, n) && ensure_alt(b.fz, n) && ensure_alt(b.vr, n) &&
-            ensure_alt(b.vphi, n) && ensure_alt(b.vz, n) && ensure_alt(b.w, n)))
-        return -3;
-      const KeysF32 keys{c.g, (const uint32_t *)a.cell, dead_key};
-      hipLaunchKernelGGL(k_count_runs<KeysF32>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist);
-      exclusive_scan(hist, hist_n, tmp, c.stream);
-      hipLaunchKernelGGL(k_place_runs<KeysF32>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist, ka);
-      in = Arrays7{{a.cell, a.fr, a.fz, a.vr, a.vphi, a.vz, a.w}};
-      out = Arrays7{{b.cell, b.fr, b.fz, b.vr, b.vphi, b.vz, b.w}};
-      hipLaunchKernelGGL(k_gather_f32, dim3(gb), dim3(256), 0, c.stream, in, out, ka, n);
-      std::swap(a.cell, b.cell);
-      std::swap(a.fr, b.fr);
-      std::swap(a.fz, b.fz);
-      std::swap(a.vr, b.vr);
-      std::swap(a.vphi, b.vphi);
-      std::swap(a.vz, b.vz);
-      std::swap(a.w, b.w);
-    } else {
-      SpeciesF64 &a = sp.f64, &b = sp.f64_alt;
-      if (!(ensure_alt(b.r, n) && ensure_alt(b.z, n) && ensure_alt(b.vr, n) && ensure_alt(b.vphi, n) &&
-            ensure_alt(b.vz, n) && ensure_alt(b.w, n) && ensure_alt(b.alive, n)))
-        return -3;
-      const KeysF64 keys{c.g, a.r, a.z, a.alive, dead_key};
-      hipLaunchKernelGGL(k_count_runs<KeysF64>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist);
-      exclusive_scan(hist, hist_n, tmp, c.stream);
-      hipLaunchKernelGGL(k_place_runs<KeysF64>, dim3(eb), dim3(256), 0, c.stream, keys, n, hist, ka);
-      in = Arrays7{{a.r, a.z, a.vr, a.vphi, a.vz, a.w, a.alive}};
-      out = Arrays7{{b.r, b.z, b.vr, b.vphi, b.vz, b.w, b.alive}};
-      hipLaunchKernelGGL(k_gather_f64, dim3(gb), dim3(256), 0, c.stream, in, out, ka, n);
-      std::swap(a.r, b.r);
-      std::swap(a.z, b.z);
-      std::swap(a.vr, b.vr);
-      std::swap(a.vphi, b.vphi);
-      std::swap(a.vz, b.vz);
-      std::swap(a.w, b.w);
-      std::swap(a.alive, b.alive);
+    return true;
+  };
+  if (!ensure_ws(fixed + 5 * (n / 32 + 1024))) return -3;
+  uint32_t *w = (uint32_t *)c.sort_ws;
+  uint32_t *src = w, *block = w + n, *tmp = block + nb + 1;
+  const unsigned eb = (unsigned)((n + 255) / 256);
+  const KeysF32 k32{c.g, (const uint32_t *)sp.f32.cell, dead_key};
+  const KeysF64 k64{c.g, sp.f64.r, sp.f64.z, sp.f64.alive, dead_key};
+  const bool f32 = c.precision == 1;
+  // runs per block, their scan (block[nb] = the number of runs)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, c.stream, block + nb, 1LL, 0u);
+  if (f32)
+    hipLaunchKernelGGL(k_run_count<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block);
+  else
+    hipLaunchKernelGGL(k_run_count<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block);
+  exclusive_scan(block, nb + 1, tmp, c.stream);
+  if (hipMemcpyAsync(c.err_host + 1, block + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+      hipStreamSynchronize(c.stream) != hipSuccess)
+    return -3;
+  const long long m = (uint32_t)c.err_host[1];
+  const long long rec_hist_m = (long long)kBins * ((m + kPerBlock - 1) / kPerBlock);
+  if (m > n) return -3;  // cannot happen: every run holds a particle
+  if (c.sort_ws_bytes < (size_t)(fixed + 5 * m) * sizeof(uint32_t)) {
+    // grow, keeping the block counts (copied over)
+    void *old = c.sort_ws;
+    c.sort_ws = nullptr;
+    c.sort_ws_bytes = 0;
+    if (!ensure_ws(fixed + 5 * m)) {
+      (void)hipFree(old);
+      return -3;
     }
-    return 0;
+    if (hipMemcpyAsync(c.sort_ws, old, (size_t)(n + nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                       c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+      (void)hipFree(old);
+      return -3;
+    }
+    (void)hipFree(old);
+    w = (uint32_t *)c.sort_ws;
+    src = w;
+    block = w + n;
+    tmp = block + nb + 1;
   }
+  uint32_t *rk = tmp + scratch, *ri = rk + m, *rk2 = ri + m, *ri2 = rk2 + m, *rlen = ri2 + m;
+  if (f32)
+    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, rk, ri, rlen);
+  else
+    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, rk, ri, rlen);
+  // records by key (stable), then each run's first rank
+  radix_pairs(rk, ri, rk2, ri2, m, bits, tmp, tmp + rec_hist_m, c.stream);
+  const unsigned mb = (unsigned)((m + 255) / 256);
+  uint32_t *lens = rk2;  // free after the sort
+  hipLaunchKernelGGL(k_run_lens, dim3(mb), dim3(256), 0, c.stream, ri, rlen, m, lens);
+  exclusive_scan(lens, m, tmp, c.stream);
+  hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, c.stream, ri, lens, m, rlen);
+  if (f32)
+    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, rlen, src);
+  else
+    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, rlen, src);
+  // all arrays in one gather
+  Arrays7 in{}, out{};
   if (c.precision == 1) {
-    if (!sp.f32.cell) return 0;
-    hipLaunchKernelGGL(k_keys_f32, dim3(eb), dim3(256), 0, c.stream, c.g, (const uint32_t *)sp.f32.cell, n, ka, va,
-                       dead_key);
+    SpeciesF32 &a = sp.f32, &b = sp.f32_alt;
+    void **ap[7] = {(void **)&a.cell, (void **)&a.fr, (void **)&a.fz, (void **)&a.vr, (void **)&a.vphi,
+                    (void **)&a.vz, (void **)&a.w};
+    void **bp[7] = {(void **)&b.cell, (void **)&b.fr, (void **)&b.fz, (void **)&b.vr, (void **)&b.vphi,
+                    (void **)&b.vz, (void **)&b.w};
+    for (int k = 0; k < 7; k++) {
+      if (!ensure_alt(*bp[k], (size_t)n * 4)) return -3;
+      in.a[k] = *ap[k];
+      out.a[k] = *bp[k];
+    }
+    hipLaunchKernelGGL(k_gather_f32, dim3(eb), dim3(256), 0, c.stream, in, out, src, n);
+    for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
   } else {
-    hipLaunchKernelGGL(k_keys_f64, dim3(eb), dim3(256), 0, c.stream, c.g, sp.f64.r, sp.f64.z, sp.f64.alive, n, ka,
-                       va, dead_key);
+    SpeciesF64 &a = sp.f64, &b = sp.f64_alt;
+    void **ap[7] = {(void **)&a.r, (void **)&a.z, (void **)&a.vr, (void **)&a.vphi, (void **)&a.vz, (void **)&a.w,
+                    (void **)&a.alive};
+    void **bp[7] = {(void **)&b.r, (void **)&b.z, (void **)&b.vr, (void **)&b.vphi, (void **)&b.vz, (void **)&b.w,
+                    (void **)&b.alive};
+    for (int k = 0; k < 7; k++) {
+      if (!ensure_alt(*bp[k], (size_t)n * (k < 6 ? 8 : 1))) return -3;
+      in.a[k] = *ap[k];
+      out.a[k] = *bp[k];
+    }
+    hipLaunchKernelGGL(k_gather_f64, dim3(eb), dim3(256), 0, c.stream, in, out, src, n);
+    for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
   }
-  for (int p = 0; p < passes; p++) {
-    int shift = 8 * p;
-    hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(kThreads), 0, c.stream, ka, n, shift, hist, nblocks);
-    exclusive_scan(hist, hist_n, tmp, c.stream);
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kThreads), 0, c.stream, ka, va, kb, vb, n, shift, hist,
-                       nblocks);
-    uint32_t *t = ka;
-    ka = kb;
-    kb = t;
-    t = va;
-    va = vb;
-    vb = t;
-  }
-  const uint32_t *perm = va;
-  bool ok = true;
-  if (c.precision == 1) {
-    ok = ok && gather_swap(sp.f32.cell, sp.f32_alt.cell, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.fr, sp.f32_alt.fr, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.fz, sp.f32_alt.fz, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.vr, sp.f32_alt.vr, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.vphi, sp.f32_alt.vphi, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.vz, sp.f32_alt.vz, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f32.w, sp.f32_alt.w, perm, n, c.stream);
-  } else {
-    ok = ok && gather_swap(sp.f64.r, sp.f64_alt.r, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.z, sp.f64_alt.z, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.vr, sp.f64_alt.vr, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.vphi, sp.f64_alt.vphi, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.vz, sp.f64_alt.vz, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.w, sp.f64_alt.w, perm, n, c.stream);
-    ok = ok && gather_swap(sp.f64.alive, sp.f64_alt.alive, perm, n, c.stream);
-  }
-  if (!ok) return -3;
   return 0;
 }
 
