@@ -88,6 +88,10 @@ void free_species(Species &sp) {
   }
   dfree(sp.inject_u);
   sp.inject_u = nullptr;
+  if (sp.runs_ev) (void)hipEventDestroy(sp.runs_ev);
+  if (sp.runs_host) (void)hipHostFree(sp.runs_host);
+  sp.runs_ev = nullptr;
+  sp.runs_host = nullptr;
 }
 
 bool is_f32(const Ctx &c) { return c.precision == PDP3_PREC_FP32; }
@@ -285,7 +289,7 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
   for (int k = 0; k < kNComp; k++) c.a[k] = c.grid + (size_t)k * n;
   if ((st = dalloc(ctx, &c.koef_e, n)) || (st = dalloc(ctx, &c.koef_h, n)) || (st = dalloc(ctx, &c.err, 1)))
     return bail(st);
-  if (hipHostMalloc((void **)&c.err_host, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&c.err_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
     return bail(PDP3_ERR_HIP);
   *c.err_host = 0;
   if (hipMemsetAsync(c.grid, 0, n * kNComp * sizeof(double), c.stream) != hipSuccess) return bail(PDP3_ERR_HIP);
@@ -843,13 +847,15 @@ int pdp3_bunch_inject(pdp3_ctx *ctx, int index) {
 }  // extern "C"
 
 namespace {
+int sort_plasma(pdp3_ctx *ctx, bool exact);
+
 // one loop iteration (src/loadInitParam.cpp:471-587); after_h (may be null)
 // runs right after the H update, when E^n and H at the half time are final
 int step_one(pdp3_ctx *ctx, const std::function<int()> *after_h) {
   Ctx &c = ctx->c;
   int st;
   if (c.sort_interval > 0 && c.steps_done > 0 && c.steps_done % c.sort_interval == 0) {
-    if ((st = pdp3_sort_particles(ctx))) return st;
+    if ((st = sort_plasma(ctx, false))) return st;
   }
   if ((st = init_beam(ctx))) return st;
   if ((st = pdp3_hfield_calc(ctx))) return st;
@@ -881,18 +887,25 @@ int pdp3_sync(pdp3_ctx *ctx) {
   return check_device_errors(ctx);
 }
 
-int pdp3_sort_particles(pdp3_ctx *ctx) {
-  if (!ctx) return PDP3_ERR_BAD_ARG;
+namespace {
+// exact: see sort_species; the periodic sorts of the loop are not exact
+int sort_plasma(pdp3_ctx *ctx, bool exact) {
   Ctx &c = ctx->c;
   TimedScope ts(c, PDP3_KERNEL_SORT);
   // bunches are injected by particle index (src/bunch.cpp:106-117) and stay unsorted
   for (int j : c.order) {
     if (c.species[j].kind != PDP3_SPECIES_PLASMA) continue;
-    int st = sort_species(c, c.species[j]);
+    int st = sort_species(c, c.species[j], exact);
     if (st) return fail(ctx, st == -3 ? PDP3_ERR_OOM : PDP3_ERR_HIP, "sort failed");
   }
   CHECK_LAUNCH(ctx);
   return PDP3_OK;
+}
+}  // namespace
+
+int pdp3_sort_particles(pdp3_ctx *ctx) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  return sort_plasma(ctx, true);
 }
 
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval) {
@@ -1361,9 +1374,11 @@ int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     if (e->sort_interval > 0 && e->steps_done > 0 && e->steps_done % e->sort_interval == 0) {
       ENS_HIP(e, hipStreamSynchronize(e->stream));
-      for (pdp3_ctx *ctx : e->ctxs) {
-        if ((st = pdp3_sort_particles(ctx)) || (st = pdp3_sync(ctx))) return efail(e, st, ctx->c.last_error);
-      }
+      // every instance's sort is enqueued on its own stream before any is waited for
+      for (pdp3_ctx *ctx : e->ctxs)
+        if ((st = sort_plasma(ctx, false))) return efail(e, st, ctx->c.last_error);
+      for (pdp3_ctx *ctx : e->ctxs)
+        if ((st = pdp3_sync(ctx))) return efail(e, st, ctx->c.last_error);
       if (ens_particles_update(e->parts, e->cs) != 0) return efail(e, PDP3_ERR_HIP, "ensemble tables");
     }
     if ((st = ens_graph(e))) return st;

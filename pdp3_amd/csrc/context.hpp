@@ -97,6 +97,11 @@ struct Species {
   // sort scratch (second buffer of the same layout) and keys
   SpeciesF64 f64_alt;
   SpeciesF32 f32_alt;
+  // sort: capacity for run records, and the previous sort's run count (pinned,
+  // copied back asynchronously; complete once runs_ev has fired)
+  long long rec_cap = 0;
+  uint32_t *runs_host = nullptr;
+  hipEvent_t runs_ev = nullptr;
   // bunch
   double *inject_u = nullptr;
   double duration = 0, hole_duration = 0, velocity = 0;
@@ -126,7 +131,7 @@ struct Ctx {
   std::vector<int> order;  // ParticlesList order of created species
   int next_bunch = 0;
   int *err = nullptr;      // device error bits
-  int *err_host = nullptr; // pinned: [0] mirror of err, [1] the sort's run count
+  int *err_host = nullptr; // pinned mirror
   std::string last_error;
   bool profiling = false;
   KernelTimer timers[5];
@@ -211,7 +216,10 @@ void launch_to_logical(Ctx &c, const double *phys, double *logical, long long n)
 void launch_alive_to_phys(Ctx &c, const uint8_t *logical, uint8_t *phys, long long n, long long n_cap);
 void launch_alive_to_logical(Ctx &c, const uint8_t *phys, uint8_t *logical, long long n);
 
-// sort
-int sort_species(Ctx &c, Species &sp);
+// cell sort of a plasma species.  exact: always sorts (record space for one
+// run per particle); otherwise the record space follows the previous sort's
+// run count, and a sort that finds more runs than that is skipped (the state
+// is left in its order) and sizes the next one
+int sort_species(Ctx &c, Species &sp, bool exact);
 
 }  // namespace pdp3

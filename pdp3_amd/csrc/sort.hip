@@ -34,9 +34,18 @@ constexpr int kPerWave = 4096;
 constexpr int kPerBlock = kWaves * kPerWave;  // 16384
 constexpr int kBins = 256;
 
-__global__ void __launch_bounds__(kThreads) k_hist(const uint32_t *__restrict__ keys, long long n, int shift,
-                                                   uint32_t *__restrict__ hist, int nblocks) {
+// the number of records present: the count on the device, or none when it
+// exceeds the capacity (that sort is then skipped)
+__device__ __forceinline__ long long present(const uint32_t *count, long long cap) {
+  const long long m = *count;
+  return m <= cap ? m : 0;
+}
+
+__global__ void __launch_bounds__(kThreads) k_hist(const uint32_t *__restrict__ keys, const uint32_t *count,
+                                                   long long cap, int shift, uint32_t *__restrict__ hist,
+                                                   int nblocks) {
   __shared__ uint32_t cnt[kBins];
+  const long long n = present(count, cap);
   cnt[threadIdx.x] = 0;
   __syncthreads();
   long long base = (long long)blockIdx.x * kPerBlock;
@@ -85,9 +94,10 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
 
 __global__ void __launch_bounds__(kThreads) k_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                      long long n, int shift, const uint32_t *__restrict__ offs,
-                                                      int nblocks) {
+                                                      const uint32_t *count, long long cap, int shift,
+                                                      const uint32_t *__restrict__ offs, int nblocks) {
   __shared__ uint32_t run[kWaves][kBins];
+  const long long n = present(count, cap);
   const int wave = threadIdx.x / 64;
   for (int t = threadIdx.x; t < kWaves * kBins; t += kThreads) run[t / kBins][t % kBins] = 0;
   __syncthreads();
@@ -189,8 +199,9 @@ __device__ __forceinline__ uint32_t run_index(uint64_t heads, int head, const ui
 
 template <class K>
 __global__ void __launch_bounds__(256) k_run_emit(K keys, long long n, const uint32_t *__restrict__ block_base,
-                                                  uint32_t *__restrict__ rkey, uint32_t *__restrict__ ridx,
-                                                  uint32_t *__restrict__ rlen) {
+                                                  const uint32_t *count, long long cap, uint32_t *__restrict__ rkey,
+                                                  uint32_t *__restrict__ ridx, uint32_t *__restrict__ rlen) {
+  if (*count > cap) return;  // more runs than record slots: this sort is skipped
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   const bool valid = s < n;
   const uint32_t key = valid ? keys(phys_of(s)) : 0u;
@@ -206,23 +217,31 @@ __global__ void __launch_bounds__(256) k_run_emit(K keys, long long n, const uin
 
 // lens[t] = length of the t-th run in key order; after the scan of lens,
 // rfirst[run] = its first rank
-__global__ void k_run_lens(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rlen, long long m,
-                           uint32_t *__restrict__ lens) {
+__global__ void k_run_lens(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rlen,
+                           const uint32_t *count, long long cap, uint32_t *__restrict__ lens) {
+  const long long m = present(count, cap);
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < m) lens[t] = rlen[order[t]];
+  if (t < cap) lens[t] = t < m ? rlen[order[t]] : 0u;
 }
-__global__ void k_run_first(const uint32_t *__restrict__ order, const uint32_t *__restrict__ lens, long long m,
-                            uint32_t *__restrict__ rfirst) {
+__global__ void k_run_first(const uint32_t *__restrict__ order, const uint32_t *__restrict__ lens,
+                            const uint32_t *count, long long cap, uint32_t *__restrict__ rfirst) {
+  const long long m = present(count, cap);
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) rfirst[order[t]] = lens[t];
 }
 
 // src[slot of the particle's rank] = its slot
+// (a skipped sort places every particle in its own slot)
 template <class K>
 __global__ void __launch_bounds__(256) k_run_place(K keys, long long n, const uint32_t *__restrict__ block_base,
+                                                   const uint32_t *count, long long cap,
                                                    const uint32_t *__restrict__ rfirst, uint32_t *__restrict__ src) {
   const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
   const bool valid = s < n;
+  if (*count > cap) {
+    if (valid) src[phys_of(s)] = (uint32_t)phys_of(s);
+    return;
+  }
   int head, len;
   const uint64_t heads = wave_runs(valid ? keys(phys_of(s)) : 0u, valid, head, len);
   const uint32_t r = run_index(heads, head, block_base);
@@ -299,16 +318,17 @@ long long scan_scratch(long long n) {
 
 // stable LSD radix sort of m (key, val) pairs by the low `bits` bits of the
 // key; returns the buffers holding the result (a or b)
-void radix_pairs(uint32_t *&ka, uint32_t *&va, uint32_t *&kb, uint32_t *&vb, long long m, int bits, uint32_t *hist,
-                 uint32_t *tmp, hipStream_t st) {
+void radix_pairs(uint32_t *&ka, uint32_t *&va, uint32_t *&kb, uint32_t *&vb, const uint32_t *count, long long cap,
+                 int bits, uint32_t *hist, uint32_t *tmp, hipStream_t st) {
   const int passes = (bits + 7) / 8;
-  const int nblocks = (int)((m + kPerBlock - 1) / kPerBlock);
+  const int nblocks = (int)((cap + kPerBlock - 1) / kPerBlock);
   const long long hist_n = (long long)kBins * nblocks;
   for (int p = 0; p < passes; p++) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(kThreads), 0, st, ka, m, shift, hist, nblocks);
+    hipLaunchKernelGGL(k_hist, dim3(nblocks), dim3(kThreads), 0, st, ka, count, cap, shift, hist, nblocks);
     exclusive_scan(hist, hist_n, tmp, st);
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kThreads), 0, st, ka, va, kb, vb, m, shift, hist, nblocks);
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(kThreads), 0, st, ka, va, kb, vb, count, cap, shift, hist,
+                       nblocks);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -325,7 +345,7 @@ bool ensure_alt(void *&alt, size_t bytes) {
 
 }  // namespace
 
-int sort_species(Ctx &c, Species &sp) {
+int sort_species(Ctx &c, Species &sp, bool exact) {
   const long long n = sp.n_cap;
   if (n <= 1 || !sp.chunked) return 0;
   if (c.precision == 1 && !sp.f32.cell) return 0;
@@ -334,81 +354,77 @@ int sort_species(Ctx &c, Species &sp) {
   int bits = 1;
   while ((1ull << bits) <= (unsigned long long)dead_key + 1) bits++;
   const long long nb = (n + 255) / 256;  // blocks of the run kernels
-  // workspace (u32 words): src (n); block counts (nb + 1); scan scratch;
-  // then the records (5 per run, sized after counting them)
-  // scratch: the scans of the block counts and of the run lengths, and the
-  // records' digit histograms with their scan (at most n records)
-  const long long rec_hist = (long long)kBins * ((n + kPerBlock - 1) / kPerBlock);
+  // record capacity: grown from the previous sort's run count (read back
+  // asynchronously), so that no sort waits for the device
+  if (!sp.runs_host) {
+    if (hipHostMalloc((void **)&sp.runs_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      sp.runs_host = nullptr;
+      return -3;
+    }
+    *sp.runs_host = 0;
+    if (hipEventCreateWithFlags(&sp.runs_ev, hipEventDisableTiming) != hipSuccess) {
+      sp.runs_ev = nullptr;
+      return -3;
+    }
+  } else if (hipEventSynchronize(sp.runs_ev) != hipSuccess) {
+    return -3;
+  }
+  const long long last = *sp.runs_host;
+  long long cap = sp.rec_cap > 0 ? sp.rec_cap : std::min(n, n / 4 + 65536);
+  if (4 * last > 3 * cap) cap = std::min(n, 2 * last + 65536);
+  sp.rec_cap = cap;
+  if (exact) cap = n;  // room for one run per particle: this sort cannot be skipped
+  // workspace (u32 words): src (n); block counts (nb + 1); scratch for the
+  // scans of the block counts and run lengths and for the records' digit
+  // histograms with their scan; the records (5 per slot)
+  const long long rec_hist = (long long)kBins * ((cap + kPerBlock - 1) / kPerBlock);
   const long long scratch =
-      std::max(std::max(scan_scratch(nb + 1), scan_scratch(n)), rec_hist + scan_scratch(rec_hist)) + 1024;
-  const long long fixed = n + (nb + 1) + scratch;
-  auto ensure_ws = [&](long long words) -> bool {
-    const size_t need = (size_t)words * sizeof(uint32_t);
-    if (c.sort_ws_bytes >= need) return true;
+      std::max(std::max(scan_scratch(nb + 1), scan_scratch(cap)), rec_hist + scan_scratch(rec_hist)) + 1024;
+  const size_t need = (size_t)(n + (nb + 1) + scratch + 5 * cap) * sizeof(uint32_t);
+  if (c.sort_ws_bytes < need) {
     if (c.sort_ws) (void)hipFree(c.sort_ws);
     c.sort_ws = nullptr;
     c.sort_ws_bytes = 0;
-    if (hipMalloc(&c.sort_ws, need) != hipSuccess) return false;
+    if (hipMalloc(&c.sort_ws, need) != hipSuccess) return -3;
     c.sort_ws_bytes = need;
-    return true;
-  };
-  if (!ensure_ws(fixed + 5 * (n / 32 + 1024))) return -3;
-  uint32_t *w = (uint32_t *)c.sort_ws;
-  uint32_t *src = w, *block = w + n, *tmp = block + nb + 1;
+  }
+  uint32_t *src = (uint32_t *)c.sort_ws, *block = src + n, *tmp = block + nb + 1;
+  uint32_t *rk = tmp + scratch, *ri = rk + cap, *rk2 = ri + cap, *ri2 = rk2 + cap, *rlen = ri2 + cap;
+  const uint32_t *count = block + nb;  // the number of runs, after the scan
   const unsigned eb = (unsigned)((n + 255) / 256);
   const KeysF32 k32{c.g, (const uint32_t *)sp.f32.cell, dead_key};
   const KeysF64 k64{c.g, sp.f64.r, sp.f64.z, sp.f64.alive, dead_key};
   const bool f32 = c.precision == 1;
-  // runs per block, their scan (block[nb] = the number of runs)
+  // runs per block and their scan (block[nb] = the number of runs)
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, c.stream, block + nb, 1LL, 0u);
   if (f32)
     hipLaunchKernelGGL(k_run_count<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block);
   else
     hipLaunchKernelGGL(k_run_count<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block);
   exclusive_scan(block, nb + 1, tmp, c.stream);
-  if (hipMemcpyAsync(c.err_host + 1, block + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-      hipStreamSynchronize(c.stream) != hipSuccess)
-    return -3;
-  const long long m = (uint32_t)c.err_host[1];
-  const long long rec_hist_m = (long long)kBins * ((m + kPerBlock - 1) / kPerBlock);
-  if (m > n) return -3;  // cannot happen: every run holds a particle
-  if (c.sort_ws_bytes < (size_t)(fixed + 5 * m) * sizeof(uint32_t)) {
-    // grow, keeping the block counts (copied over)
-    void *old = c.sort_ws;
-    c.sort_ws = nullptr;
-    c.sort_ws_bytes = 0;
-    if (!ensure_ws(fixed + 5 * m)) {
-      (void)hipFree(old);
-      return -3;
-    }
-    if (hipMemcpyAsync(c.sort_ws, old, (size_t)(n + nb + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                       c.stream) != hipSuccess ||
-        hipStreamSynchronize(c.stream) != hipSuccess) {
-      (void)hipFree(old);
-      return -3;
-    }
-    (void)hipFree(old);
-    w = (uint32_t *)c.sort_ws;
-    src = w;
-    block = w + n;
-    tmp = block + nb + 1;
-  }
-  uint32_t *rk = tmp + scratch, *ri = rk + m, *rk2 = ri + m, *ri2 = rk2 + m, *rlen = ri2 + m;
   if (f32)
-    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, rk, ri, rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, count, cap, rk,
+                       ri, rlen);
   else
-    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, rk, ri, rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, count, cap, rk,
+                       ri, rlen);
   // records by key (stable), then each run's first rank
-  radix_pairs(rk, ri, rk2, ri2, m, bits, tmp, tmp + rec_hist_m, c.stream);
-  const unsigned mb = (unsigned)((m + 255) / 256);
+  radix_pairs(rk, ri, rk2, ri2, count, cap, bits, tmp, tmp + rec_hist, c.stream);
+  const unsigned mb = (unsigned)((cap + 255) / 256);
   uint32_t *lens = rk2;  // free after the sort
-  hipLaunchKernelGGL(k_run_lens, dim3(mb), dim3(256), 0, c.stream, ri, rlen, m, lens);
-  exclusive_scan(lens, m, tmp, c.stream);
-  hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, c.stream, ri, lens, m, rlen);
+  hipLaunchKernelGGL(k_run_lens, dim3(mb), dim3(256), 0, c.stream, ri, rlen, count, cap, lens);
+  exclusive_scan(lens, cap, tmp, c.stream);
+  hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, c.stream, ri, lens, count, cap, rlen);
   if (f32)
-    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, count, cap,
+                       rlen, src);
   else
-    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, count, cap,
+                       rlen, src);
+  // the run count for the next sort's capacity
+  if (hipMemcpyAsync(sp.runs_host, count, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+      hipEventRecord(sp.runs_ev, c.stream) != hipSuccess)
+    return -3;
   // all arrays in one gather
   Arrays7 in{}, out{};
   if (c.precision == 1) {

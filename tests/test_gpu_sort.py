@@ -65,3 +65,43 @@ def test_sorted_step_matches_unsorted():
     for name in ("E_r", "E_z", "H_phi", "J_r", "J_z", "J_phi"):
         e = normwise(b.field(name), a.field(name))
         assert e <= 1e-9, (name, e)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_periodic_sort_of_unsorted_state(precision):
+    """Periodic sorts in the loop size their record space from the previous
+    sort: on a shuffled state the first finds more runs than that space and is
+    skipped (the state keeps its order), the next ones sort.  The fields and
+    the particles match the unsorted run; an explicit sort afterwards orders
+    the state by cell."""
+    from pdp3_amd import capi
+
+    rng = np.random.default_rng(33)
+    nr, nz, dr = 33, 129, 5e-4
+    sims = []
+    for si in (0, 2):
+        sim = capi.Simulation(nr, nz, 32 * dr, 128 * dr, 0.5 * dr / 3e8, precision=precision)
+        sims.append(sim)
+    n = 200000
+    r = rng.uniform(dr / 2, 31.5 * dr, n)
+    z = rng.uniform(dr / 2, 127.5 * dr, n)
+    v = rng.standard_normal((3, n)) * 1e5
+    w = rng.uniform(1, 2, n)
+    for sim, si in zip(sims, (0, 2)):
+        sim.boundary_conditions()
+        sim.add_plasma(-1, 1.0, r, z, v[0], v[1], v[2], w)
+        sim.set_sort_interval(si)
+        sim.step(7)  # sorts before steps 3, 5 and 7
+    for name in ("E_r", "E_z", "H_phi", "J_r", "J_z", "J_phi"):
+        e = normwise(sims[1].field(name), sims[0].field(name))
+        assert e <= 1e-9, (name, e)
+    a, b = sims[0].download(0), sims[1].download(0)
+    oa, ob = np.lexsort((a["z"], a["r"])), np.lexsort((b["z"], b["r"]))
+    for k in ("r", "z", "vr", "vphi", "vz", "w"):
+        assert normwise(b[k][ob], a[k][oa]) <= 1e-9, k
+    sims[1].step(1)
+    sims[1].set_sort_interval(0)
+    sims[1].sort()
+    st = sims[1].download(0)
+    key = _cell_keys(st, sims[1].dr, sims[1].dz, nr, nz)
+    assert np.all(np.diff(key) >= 0)
