@@ -75,19 +75,12 @@ struct RadW {
 // (l = its staggered index, i or i-1) and the own family's 2i, 2(i+2) are all
 // entries of that row.
 __device__ __forceinline__ void rad_weights(int i, float a, const float4 &t, RadW &lo, RadW &own) {
-  int ir = i;
-  float al = a - 0.5f;
-  bool down = false;
-  if (al <= 0.f) {
-    ir = i - 1;
-    al = a + 0.5f;
-    down = true;
-  }
-  if (ir < 0) {  // clamped at the axis as the reference does
-    ir = 0;
-    al = (float)i + a - 0.5f;
-    down = false;
-  }
+  // staggered index from r - dr/2: i-1 with offset a+1/2 when a <= 1/2, except
+  // on the axis cell, where the reference clamps the index at 0 and keeps the
+  // offset a-1/2
+  const bool down = a <= 0.5f && i > 0;
+  const int ir = down ? i - 1 : i;
+  const float al = down ? a + 0.5f : a - 0.5f;
   const float fl = (float)ir, fi = (float)i;
   lo.ir = ir;
   lo.A = (1.f - al) * (2.f * fl + 1.f + al) * (down ? t.x : t.y);
@@ -107,18 +100,10 @@ struct AxW {
   float b;  // weight of node kz+1; node kz gets 1-b
 };
 __device__ __forceinline__ AxW ax_own(int k, float b) { return AxW{k, b}; }
+// staggered axial index from z - dz/2 (clamped at 0 as the reference does)
 __device__ __forceinline__ AxW ax_half(int k, float b) {
-  int kz = k;
-  float be = b - 0.5f;
-  if (be <= 0.f) {
-    kz = k - 1;
-    be = b + 0.5f;
-  }
-  if (kz < 0) {
-    kz = 0;
-    be = (float)k + b - 0.5f;
-  }
-  return AxW{kz, be};
+  const bool down = b <= 0.5f && k > 0;
+  return AxW{down ? k - 1 : k, down ? b + 0.5f : b - 0.5f};
 }
 
 // node values are rounded to FP32 and multiplied by the push's scale s
@@ -284,15 +269,12 @@ __device__ __forceinline__ void vay_u(const F32Args &p, const float e[3], const 
   u[2] = s * (up[2] + alpha * b[2] + b[1] * up[0] - b[0] * up[1]);
 }
 
-// keep an offset in (0, 1] by moving at most one cell
+// bring an offset back into (0, 1]: by ceil(o) - 1 cells (one cell at most
+// after a half step; exact in FP32)
 __device__ __forceinline__ void renorm(int &c, float &o) {
-  if (o > 1.f) {
-    o -= 1.f;
-    c += 1;
-  } else if (o <= 0.f) {
-    o += 1.f;
-    c -= 1;
-  }
+  const float m = ceilf(o) - 1.f;
+  o -= m;
+  c += (int)m;
 }
 
 // r of the Cartesian point (X, dy), X = i + xa, as an offset update that
@@ -418,36 +400,39 @@ __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, in
   }
   float drho = (float)di + (an - ao);
   float dzeta = (float)dk + (bn - bo);
-  // parameters along the path where it crosses the faces (in old-cell units
-  // the radial face is at offset 1 (di=+1) or 0 (di=-1))
-  float tr = 2.f, tz = 2.f;
-  if (di != 0) tr = ((di > 0 ? 1.f : 0.f) - ao) / drho;
-  if (dk != 0) tz = ((dk > 0 ? 1.f : 0.f) - bo) / dzeta;
+  // pieces before the faces the path crosses, at the path parameters tr, tz
+  // where it meets them (in old-cell units the radial face is at offset 1
+  // (di=+1) or 0 (di=-1))
   int ci = io, ck = ko;
   float ca = ao, cb = bo;  // current start point, offsets in cell (ci, ck)
-  float tprev = 0.f;
-  for (int pass = 0; pass < 2; pass++) {
-    bool rfirst = tr <= tz;
-    float t = rfirst ? tr : tz;
-    if (t > 1.f) break;
-    float sd_r = (t - tprev) * drho, sd_z = (t - tprev) * dzeta;
-    if (rfirst) {
-      float fr = di > 0 ? 1.f : 0.f;
-      segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z, rt);
-      ci += di;
-      rt = p.rows[2 * min(max(ci, 0), p.nr - 1) + 1];
-      ca = di > 0 ? 0.f : 1.f;
-      cb += sd_z;
-      tr = 2.f;
-    } else {
-      float fz = dk > 0 ? 1.f : 0.f;
-      segment(p, sk, w, ci, ck, ca, cb, sd_r, fz - cb, rt);
-      ck += dk;
-      cb = dk > 0 ? 0.f : 1.f;
-      ca += sd_r;
-      tz = 2.f;
+  if (di != 0 || dk != 0) {
+    float tr = 2.f, tz = 2.f;
+    if (di != 0) tr = fdiv((di > 0 ? 1.f : 0.f) - ao, drho);
+    if (dk != 0) tz = fdiv((dk > 0 ? 1.f : 0.f) - bo, dzeta);
+    float tprev = 0.f;
+    for (int pass = 0; pass < 2; pass++) {
+      bool rfirst = tr <= tz;
+      float t = rfirst ? tr : tz;
+      if (t > 1.f) break;
+      float sd_r = (t - tprev) * drho, sd_z = (t - tprev) * dzeta;
+      if (rfirst) {
+        float fr = di > 0 ? 1.f : 0.f;
+        segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z, rt);
+        ci += di;
+        rt = p.rows[2 * min(max(ci, 0), p.nr - 1) + 1];
+        ca = di > 0 ? 0.f : 1.f;
+        cb += sd_z;
+        tr = 2.f;
+      } else {
+        float fz = dk > 0 ? 1.f : 0.f;
+        segment(p, sk, w, ci, ck, ca, cb, sd_r, fz - cb, rt);
+        ck += dk;
+        cb = dk > 0 ? 0.f : 1.f;
+        ca += sd_r;
+        tz = 2.f;
+      }
+      tprev = t;
     }
-    tprev = t;
   }
   // last piece to the new position
   segment(p, sk, w, ci, ck, ca, cb, (an + (float)(in_ - ci)) - ca, (bn + (float)(kn - ck)) - cb, rt);
