@@ -358,6 +358,7 @@ template <class S>
 __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a, float b, float drho, float dzeta,
                         const float4 &t) {
   float jz0, jz1, jr0, jr1;
+  float f0 = 1.f, f1 = 1.f, fr = 1.f;  // factors of the J_z pair and of J_r
   if (i == 0) {
     double rs = a;  // axis cell in FP64
     jz0 = (float)(8.0 * (1.0 - 0.5 * drho - rs));
@@ -365,24 +366,30 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
     jr0 = drho * (0.5f * (1.f - b) - 0.25f * dzeta) * 2.f;
     jr1 = drho * (0.5f * b + 0.25f * dzeta) * 2.f;
   } else {
-    // t: second table entry of row i (1/i, 1/(i+1), 1/(i+1/2), i(i+1))
+    // t: second table entry of row i (1/i, 1/(i+1), 1/(i+1/2), i(i+1)); the
+    // closed forms above, with 1/rho factored out of the J_z brackets
     const float fi = (float)i;
     const float rs = fi + a, re = rs + drho;
     // 1/rs, 1/re from one reciprocal
     const float q = __builtin_amdgcn_rcpf(rs * re);
     const float irs = re * q, ire = rs * q;
-    const float ii = t.w, inv_i = t.x, inv_i1 = t.y, inv = t.z;
-    float x = drho * irs;
-    float M = m_of(x);
-    float N = -drho * ire - M;  // x / (1 + x) = drho / re
-    jz0 = ((1.f - a) * (2.f * fi + a) * irs - 0.5f * drho + ii * irs * M) * inv_i;
-    jz1 = (a * (2.f * fi + 1.f + a) * irs + 0.5f * drho - ii * irs * M) * inv_i1;
-    float cc = 0.5f * ii * irs;
-    jr0 = (drho * (0.5f * (1.f - b) - 0.25f * dzeta) + cc * (drho * (1.f - b) * ire + dzeta * N)) * inv;
-    jr1 = (drho * (0.5f * b + 0.25f * dzeta) + cc * (drho * b * ire - dzeta * N)) * inv;
+    const float ii = t.w;
+    const float M = m_of(drho * irs);
+    const float iiM = ii * M;
+    const float N = -drho * ire - M;  // x / (1 + x) = drho / re
+    const float hdr = 0.5f * drho;
+    jz0 = fmaf(irs, fmaf(1.f - a, 2.f * fi + a, iiM), -hdr);
+    jz1 = fmaf(irs, fmaf(a, 2.f * fi + 1.f + a, -iiM), hdr);
+    const float cc = 0.5f * ii * irs, qdz = 0.25f * dzeta, dri = drho * ire, dN = dzeta * N, ub = 1.f - b;
+    jr0 = fmaf(drho, fmaf(0.5f, ub, -qdz), cc * fmaf(dri, ub, dN));
+    jr1 = fmaf(drho, fmaf(0.5f, b, qdz), cc * fmaf(dri, b, -dN));
+    // the divisions by i, i+1 and i+1/2 go into the scale factors
+    f0 = t.x;
+    f1 = t.y;
+    fr = t.z;
   }
-  float sz = p.kz * w * dzeta, sr = p.kr * w;
-  sk.template add2<kCZ>(i, k, (double)(sz * jz0), (double)(sz * jz1));
+  const float sz = p.kz * w * dzeta, sr = p.kr * w * fr;
+  sk.template add2<kCZ>(i, k, (double)(jz0 * (sz * f0)), (double)(jz1 * (sz * f1)));
   sk.template add2<kCR>(i, k, (double)(sr * jr0), (double)(sr * jr1));
 }
 
