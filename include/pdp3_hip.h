@@ -186,11 +186,14 @@ int pdp3_step(pdp3_ctx *ctx, int nsteps);
 /* waits for the context's stream and reports device-side errors */
 int pdp3_sync(pdp3_ctx *ctx);
 
-/* stable counting sort of every plasma species by cell (no reference
- * counterpart: keeps gathers and deposits local; bunches keep their index
- * order, which their injection relies on).  Particle indices change: download
- * after a sort returns the sorted order.  sort_interval > 0 makes pdp3_step
- * sort every sort_interval steps; 0 disables. */
+/* stable sort of every plasma species by cell (no reference counterpart:
+ * keeps gathers and deposits local; bunches keep their index order, which
+ * their injection relies on).  Particle indices change: download after a sort
+ * returns the sorted order.  sort_interval > 0 makes pdp3_step sort every
+ * sort_interval steps; 0 disables.  The periodic sorts do not wait for the
+ * device: their record space follows the previous sort's count of runs of
+ * equal cells, and a sort that finds more runs than that leaves the order as
+ * it is (the next one is sized for it).  pdp3_sort_particles always sorts. */
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
 
@@ -251,7 +254,9 @@ int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out);
 int pdp3_ensemble_destroy(pdp3_ensemble *e);
 const char *pdp3_ensemble_last_error(const pdp3_ensemble *e);
 int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps);
-/* cell sort of every instance every sort_interval steps (outside the graph) */
+/* cell sort of every instance every sort_interval steps (outside the graph;
+ * the periodic sorts of pdp3_set_sort_interval, enqueued for all instances
+ * before any is waited for) */
 int pdp3_ensemble_set_sort_interval(pdp3_ensemble *e, int sort_interval);
 /* device time of the stepped graphs since the last call (HIP events), and the step count */
 int pdp3_ensemble_timing(pdp3_ensemble *e, double *total_ms, long long *steps);
