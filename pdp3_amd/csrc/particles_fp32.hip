@@ -990,17 +990,16 @@ int ens_blocks_of(int tile) {
 }
 constexpr long long kBunchItem = (long long)kWavesPerBlock * kChunk;
 
-// the argument tables and work lists of the ensemble's species
-void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<GlobalSink> *sinks,
-               std::vector<EnsItem> items[kNumTiles], std::vector<EnsItem> *bitems) {
+// the argument table and work lists of the ensemble's species
+void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<EnsItem> items[kNumTiles],
+               std::vector<EnsItem> *bitems) {
   int a = 0;
   for (Ctx *cp : ctxs) {
     Ctx &c = *cp;
     for (int j : c.order) {
       Species &sp = c.species[j];
       if (sp.n_cap <= 0 || !sp.f32.cell) continue;
-      if (args) args->push_back(args32(c, sp));
-      if (sinks) sinks->push_back(sink32(c));
+      args->push_back(args32(c, sp));
       if (sp.chunked) {
         for (int t = 0; t < kNumTiles; t++) {
           const long long tile = (long long)kTileShapes[t][1] * kTileShapes[t][0] / kPPT * kChunk;
@@ -1015,23 +1014,21 @@ void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::
 }
 }  // namespace
 
+// the argument tables (a sort swaps the particle arrays); the work lists
+// depend only on the species sizes and stay as built
 int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs) {
   std::vector<F32Args> args;
   std::vector<GlobalSink> sinks;
-  std::vector<EnsItem> items[kNumTiles], bitems;
-  ens_lists(ctxs, &args, &sinks, items, &bitems);
-  if ((int)args.size() != e->n_args || (int)bitems.size() != e->n_bitems) return -1;  // the species may not change
+  for (Ctx *cp : ctxs)
+    for (int j : cp->order) {
+      Species &sp = cp->species[j];
+      if (sp.n_cap <= 0 || !sp.f32.cell) continue;
+      args.push_back(args32(*cp, sp));
+      sinks.push_back(sink32(*cp));
+    }
+  if ((int)args.size() != e->n_args) return -1;  // the species may not change
   if (hipMemcpy(e->args, args.data(), args.size() * sizeof(F32Args), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(e->sinks, sinks.data(), sinks.size() * sizeof(GlobalSink), hipMemcpyHostToDevice) != hipSuccess)
-    return -1;
-  for (int t = 0; t < kNumTiles; t++) {
-    if ((int)items[t].size() != e->n_items[t]) return -1;
-    if (!items[t].empty() && hipMemcpy(e->items[t], items[t].data(), items[t].size() * sizeof(EnsItem),
-                                       hipMemcpyHostToDevice) != hipSuccess)
-      return -1;
-  }
-  if (!bitems.empty() &&
-      hipMemcpy(e->bitems, bitems.data(), bitems.size() * sizeof(EnsItem), hipMemcpyHostToDevice) != hipSuccess)
     return -1;
   return 0;
 }
@@ -1040,7 +1037,7 @@ EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
   auto *e = new EnsParticles;
   std::vector<F32Args> args;
   std::vector<EnsItem> items[kNumTiles], bitems;
-  ens_lists(ctxs, &args, nullptr, items, &bitems);
+  ens_lists(ctxs, &args, items, &bitems);
   e->n_args = (int)args.size();
   e->n_bitems = (int)bitems.size();
   bool ok = hipMalloc((void **)&e->args, std::max(1, e->n_args) * sizeof(F32Args)) == hipSuccess &&
@@ -1048,9 +1045,13 @@ EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
             hipMalloc((void **)&e->bitems, std::max(1, e->n_bitems) * sizeof(EnsItem)) == hipSuccess;
   for (int t = 0; ok && t < kNumTiles; t++) {
     e->n_items[t] = (int)items[t].size();
-    ok = hipMalloc((void **)&e->items[t], std::max<size_t>(1, items[t].size()) * sizeof(EnsItem)) == hipSuccess;
+    ok = hipMalloc((void **)&e->items[t], std::max<size_t>(1, items[t].size()) * sizeof(EnsItem)) == hipSuccess &&
+         (items[t].empty() || hipMemcpy(e->items[t], items[t].data(), items[t].size() * sizeof(EnsItem),
+                                        hipMemcpyHostToDevice) == hipSuccess);
     e->blocks[t] = ens_blocks_of(t);
   }
+  ok = ok && (bitems.empty() || hipMemcpy(e->bitems, bitems.data(), bitems.size() * sizeof(EnsItem),
+                                          hipMemcpyHostToDevice) == hipSuccess);
   e->bblocks = resident_blocks(reinterpret_cast<const void *>(&k_ens_bunch32), 256);
   if (!ok || ens_particles_update(e, ctxs) != 0) {
     ens_particles_free(e);
