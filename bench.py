@@ -234,7 +234,7 @@ def bench_adjoint(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; cfg4: 40, one sort interval)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg3")
     ap.add_argument("--sort-interval", type=int, default=None,
@@ -260,6 +260,8 @@ def main():
 
     if a.sort_interval is None:
         a.sort_interval = 40 if a.config == "cfg4" else 20
+    if a.steps is None:
+        a.steps = 40 if a.config == "cfg4" else 20
     if a.config == "cfg4":
         bench_ensemble(a)
         return

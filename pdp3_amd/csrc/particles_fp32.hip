@@ -698,10 +698,11 @@ __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs,
   fused32_bunch_tile<kWavesPerBlock>(p, gs, (long long)blockIdx.x * kBlockParticles, n_cap, box);
 }
 
-// ensembles: the plasma tiles of every instance in one persistent launch and
-// their bunch blocks in a second (two kernels keep the plasma path at the
-// register count of the single-instance kernel); blocks walk the item list
-// with a stride of the grid
+// ensembles: the plasma tiles and bunch blocks of every instance in one
+// persistent launch; blocks walk the item list with a stride of the grid.
+// Bunch blocks take the plasma tile path too (their particles are not cell
+// sorted, so most of their gathers and deposits take the boxes' HBM
+// fallbacks), which keeps the kernel at the plasma path's register count.
 struct EnsItem {
   int arg;   // index into the per-(instance, species) argument tables
   int kind;  // 0 plasma tile, 1 bunch block
@@ -710,25 +711,14 @@ struct EnsItem {
 
 template <int PPT, int WAVES>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-k_ens_plasma32(const F32Args *__restrict__ args, const GlobalSink *__restrict__ sinks,
-               const EnsItem *__restrict__ items, int n_items) {
+k_ens_fused32(const F32Args *__restrict__ args, const GlobalSink *__restrict__ sinks,
+              const EnsItem *__restrict__ items, int n_items) {
   __shared__ double box[TileSink::kSize];
   __shared__ __align__(16) float fbox[kFBoxFloats];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const EnsItem item = items[it];
     fused32_tile<PPT, WAVES>(args[item.arg], sinks[item.arg], item.base, item.n_cap, box, fbox);
     __syncthreads();  // the next item clears the boxes
-  }
-}
-
-__global__ void __launch_bounds__(256) k_ens_bunch32(const F32Args *__restrict__ args,
-                                                     const GlobalSink *__restrict__ sinks,
-                                                     const EnsItem *__restrict__ items, int n_items) {
-  __shared__ double box[BunchSink::kSize];
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const EnsItem item = items[it];
-    fused32_bunch_tile<kWavesPerBlock>(args[item.arg], sinks[item.arg], item.base, item.n_cap, box);
-    __syncthreads();  // the next item clears the box
   }
 }
 
@@ -957,26 +947,20 @@ struct EnsParticles {
   int n_args = 0;
   F32Args *args = nullptr;
   GlobalSink *sinks = nullptr;
-  // plasma tiles per tile shape; bunch blocks (one shape)
+  // work list per tile shape (plasma tiles and bunch blocks)
   EnsItem *items[kNumTiles] = {};
   int n_items[kNumTiles] = {};
   int blocks[kNumTiles] = {};
-  EnsItem *bitems = nullptr;
-  int n_bitems = 0;
-  int bblocks = 0;
 };
 
 namespace {
-int resident_blocks(const void *kernel, int threads) {
+template <int PPT, int WAVES>
+int ens_blocks() {
   int per_cu = 0, cus = 0, dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ens_fused32<PPT, WAVES>, WAVES * 64, 0);
   return std::max(1, per_cu) * std::max(1, cus);
-}
-template <int PPT, int WAVES>
-int ens_blocks() {
-  return resident_blocks(reinterpret_cast<const void *>(&k_ens_plasma32<PPT, WAVES>), WAVES * 64);
 }
 int ens_blocks_of(int tile) {
   switch (tile) {
@@ -988,11 +972,9 @@ int ens_blocks_of(int tile) {
     default: return ens_blocks<4, 4>();
   }
 }
-constexpr long long kBunchItem = (long long)kWavesPerBlock * kChunk;
 
-// the argument table and work lists of the ensemble's species
-void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<EnsItem> items[kNumTiles],
-               std::vector<EnsItem> *bitems) {
+// the argument table and the work lists of the ensemble's species
+void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<EnsItem> items[kNumTiles]) {
   int a = 0;
   for (Ctx *cp : ctxs) {
     Ctx &c = *cp;
@@ -1000,13 +982,9 @@ void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::
       Species &sp = c.species[j];
       if (sp.n_cap <= 0 || !sp.f32.cell) continue;
       args->push_back(args32(c, sp));
-      if (sp.chunked) {
-        for (int t = 0; t < kNumTiles; t++) {
-          const long long tile = (long long)kTileShapes[t][1] * kTileShapes[t][0] / kPPT * kChunk;
-          for (long long b = 0; b < sp.n_cap; b += tile) items[t].push_back(EnsItem{a, 0, b, sp.n_cap});
-        }
-      } else {
-        for (long long b = 0; b < sp.n_cap; b += kBunchItem) bitems->push_back(EnsItem{a, 1, b, sp.n_cap});
+      for (int t = 0; t < kNumTiles; t++) {
+        const long long tile = (long long)kTileShapes[t][1] * kTileShapes[t][0] / kPPT * kChunk;
+        for (long long b = 0; b < sp.n_cap; b += tile) items[t].push_back(EnsItem{a, sp.chunked ? 0 : 1, b, sp.n_cap});
       }
       a++;
     }
@@ -1036,13 +1014,11 @@ int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs) {
 EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
   auto *e = new EnsParticles;
   std::vector<F32Args> args;
-  std::vector<EnsItem> items[kNumTiles], bitems;
-  ens_lists(ctxs, &args, items, &bitems);
+  std::vector<EnsItem> items[kNumTiles];
+  ens_lists(ctxs, &args, items);
   e->n_args = (int)args.size();
-  e->n_bitems = (int)bitems.size();
   bool ok = hipMalloc((void **)&e->args, std::max(1, e->n_args) * sizeof(F32Args)) == hipSuccess &&
-            hipMalloc((void **)&e->sinks, std::max(1, e->n_args) * sizeof(GlobalSink)) == hipSuccess &&
-            hipMalloc((void **)&e->bitems, std::max(1, e->n_bitems) * sizeof(EnsItem)) == hipSuccess;
+            hipMalloc((void **)&e->sinks, std::max(1, e->n_args) * sizeof(GlobalSink)) == hipSuccess;
   for (int t = 0; ok && t < kNumTiles; t++) {
     e->n_items[t] = (int)items[t].size();
     ok = hipMalloc((void **)&e->items[t], std::max<size_t>(1, items[t].size()) * sizeof(EnsItem)) == hipSuccess &&
@@ -1050,9 +1026,6 @@ EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
                                         hipMemcpyHostToDevice) == hipSuccess);
     e->blocks[t] = ens_blocks_of(t);
   }
-  ok = ok && (bitems.empty() || hipMemcpy(e->bitems, bitems.data(), bitems.size() * sizeof(EnsItem),
-                                          hipMemcpyHostToDevice) == hipSuccess);
-  e->bblocks = resident_blocks(reinterpret_cast<const void *>(&k_ens_bunch32), 256);
   if (!ok || ens_particles_update(e, ctxs) != 0) {
     ens_particles_free(e);
     return nullptr;
@@ -1064,7 +1037,6 @@ void ens_particles_free(EnsParticles *e) {
   if (!e) return;
   if (e->args) (void)hipFree(e->args);
   if (e->sinks) (void)hipFree(e->sinks);
-  if (e->bitems) (void)hipFree(e->bitems);
   for (auto *p : e->items)
     if (p) (void)hipFree(p);
   delete e;
@@ -1074,7 +1046,7 @@ template <int PPT, int WAVES>
 void ens_launch_t(EnsParticles *e, int t, hipStream_t st) {
   const int blocks = std::min(e->n_items[t], e->blocks[t]);
   if (blocks <= 0) return;
-  hipLaunchKernelGGL((k_ens_plasma32<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, e->args, e->sinks,
+  hipLaunchKernelGGL((k_ens_fused32<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, e->args, e->sinks,
                      e->items[t], e->n_items[t]);
 }
 
@@ -1087,9 +1059,6 @@ void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st) {
     case 4: ens_launch_t<8, 2>(e, 4, st); break;
     default: ens_launch_t<4, 4>(e, 5, st); break;
   }
-  const int bb = std::min(e->n_bitems, e->bblocks);
-  if (bb > 0)
-    hipLaunchKernelGGL(k_ens_bunch32, dim3(bb), dim3(256), 0, st, e->args, e->sinks, e->bitems, e->n_bitems);
 }
 
 }  // namespace pdp3
