@@ -63,6 +63,39 @@ def run_form(wl, capi, w, precision, steps, warmup, sort_interval):
     return out, ez
 
 
+def momentum_error(wl, capi, name, cells_z, steps):
+    """Particle momentum (gamma v) and E_z errors of the FP32 form against the
+    FP64 plain form on a z-slab of the workload: the same spacing, ppc, density
+    profile and bunch over the first cells_z cells, no cell sort (both runs keep
+    one particle order, so particles pair up by index)."""
+    w = wl.config(name)
+    w.cells_z = cells_z
+    sims = [wl.build_simulation(w, precision=p) for p in (capi.PREC_FP32, capi.PREC_FP64)]
+    for s_ in sims:
+        s_.step(steps)
+    ez = [s_.field("E_z") for s_ in sims]
+    mom = []
+    for s_ in sims:
+        d = s_.download(0)
+        g = 1.0 / np.sqrt(1.0 - (d["vr"] ** 2 + d["vphi"] ** 2 + d["vz"] ** 2) / wl.LIGHT_SPEED ** 2)
+        mom.append(np.stack([g * d["vr"], g * d["vphi"], g * d["vz"]]))
+        del d
+        s_.close()
+    dp = mom[0] - mom[1]
+    # per particle |dp| / rms |p|; the largest come from particles whose wall or
+    # axis reflection fell on another step in one of the forms (a sign flip)
+    rms_p = np.sqrt(np.mean(np.sum(mom[1] ** 2, axis=0)))
+    per = np.sqrt(np.sum(dp ** 2, axis=0)) / rms_p
+    return dict(momentum_rel_rms=float(np.sqrt(np.sum(dp ** 2) / np.sum(mom[1] ** 2))),
+                momentum_rel_max=float(np.max(np.abs(dp)) / np.max(np.abs(mom[1]))),
+                momentum_per_particle_p9999=float(np.quantile(per, 0.9999)),
+                particles_above_1e_3=int(np.count_nonzero(per > 1e-3)),
+                E_z_rel_rms=float(np.sqrt(np.mean((ez[0] - ez[1]) ** 2) / np.mean(ez[1] ** 2))),
+                E_z_rel_max=float(np.max(np.abs(ez[0] - ez[1])) / np.max(np.abs(ez[1]))),
+                sample=f"{w.cells_r}x{cells_z}-cell z-slab of {name}, {mom[1].shape[1]} plasma particles, "
+                       f"{steps} steps, no cell sort")
+
+
 def cpu_baseline(wl, capi, w, threads):
     """The oracle (FP64, reference-faithful: push/move loops threaded, deposits
     serial as in the reference) on a bounded sample of the workload: the same
@@ -294,6 +327,8 @@ def main():
         err = dict(E_z_rel_rms=float(np.sqrt(np.mean((ez32 - ez64) ** 2)) / den),
                    E_z_rel_max=float(np.max(np.abs(ez32 - ez64)) / np.max(np.abs(ez64))),
                    after_steps=a.steps + a.warmup, reference="FP64 plain form on the GPU (oracle-parity checked)")
+        err["particles"] = momentum_error(wl, capi, a.config, min(1024, w.cells_z), a.steps + a.warmup)
+        print(f"# particle errors: {err['particles']}", file=sys.stderr, flush=True)
         p64 = r64["particle_ms"] / max(r64["particle_launches"], 1)
         fp64 = dict(value=v64, ms_per_step=1e3 * r64["wall_s"] / a.steps, particle_kernel_ms=p64,
                     roofline_frac=(n_plasma * BYTES_PER_PARTICLE["fp64"] + cells * BYTES_PER_CELL) / (p64 * 1e-3)
