@@ -527,11 +527,35 @@ __device__ __forceinline__ PState load_state(const F32Args &p, unsigned off) {
   return s;
 }
 
-// the whole step of one particle (state ps, slot at byte offset off)
+// the state for the lane's next iteration: plain loads (they are waited for
+// where the compiler needs them, or by wait_loads)
+__device__ __forceinline__ PState load_state_next(const F32Args &p, unsigned off) {
+  PState s;
+  s.c = __builtin_nontemporal_load(&at(p.cell, off));
+  s.a = __builtin_nontemporal_load(&at(p.fa, off));
+  s.b = __builtin_nontemporal_load(&at(p.fb, off));
+  s.u0 = __builtin_nontemporal_load(&at(p.ur, off));
+  s.u1 = __builtin_nontemporal_load(&at(p.uphi, off));
+  s.u2 = __builtin_nontemporal_load(&at(p.uz, off));
+  s.w = __builtin_nontemporal_load(&at(p.w, off));
+  return s;
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): every vector
+// memory access of the wave has completed
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// the new state of one particle
+struct POut {
+  uint32_t c;
+  float a, b, u0, u1, u2;
+};
+
+// the whole step of one particle (state ps) without its stores: false for a
+// dead slot (nothing to store)
 template <class S, class FS>
-__device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &fs, const PState &ps, unsigned off) {
+__device__ __forceinline__ bool step_state32(const F32Args &p, S &sk, const FS &fs, const PState &ps, POut &o) {
   const uint32_t c = ps.c;
-  if (c & kDead) return;
+  if (c & kDead) return false;
   int i = cell_i(c), k = cell_k(c);
   float a = ps.a, b = ps.b;
   float u[3] = {ps.u0, ps.u1, ps.u2};
@@ -575,23 +599,34 @@ __device__ __forceinline__ void step_state32(const F32Args &p, S &sk, const FS &
     u[1] = u1;
     if (p.want_rho) deposit_rho(p, sk, w, i1, xa2, k1, b2);
   }
+  o.c = pack(i1, k1) | (alive ? 0u : kDead);
+  o.a = xa2;
+  o.b = b2;
+  o.u0 = u[0];
+  o.u1 = u[1];
+  o.u2 = u[2];
+  return true;
+}
+
+__device__ __forceinline__ void store_state(const F32Args &p, const POut &o, unsigned off) {
   // a fresh copy of the offset: the stores then address from the scalar bases
   // too, instead of reusing 64-bit addresses formed for the loads
   unsigned so = off;
   asm("" : "+v"(so));
-  __builtin_nontemporal_store(pack(i1, k1) | (alive ? 0u : kDead), &at(p.cell, so));
-  __builtin_nontemporal_store(xa2, &at(p.fa, so));
-  __builtin_nontemporal_store(b2, &at(p.fb, so));
-  __builtin_nontemporal_store(u[0], &at(p.ur, so));
-  __builtin_nontemporal_store(u[1], &at(p.uphi, so));
-  __builtin_nontemporal_store(u[2], &at(p.uz, so));
+  __builtin_nontemporal_store(o.c, &at(p.cell, so));
+  __builtin_nontemporal_store(o.a, &at(p.fa, so));
+  __builtin_nontemporal_store(o.b, &at(p.fb, so));
+  __builtin_nontemporal_store(o.u0, &at(p.ur, so));
+  __builtin_nontemporal_store(o.u1, &at(p.uphi, so));
+  __builtin_nontemporal_store(o.u2, &at(p.uz, so));
 }
 
 // the whole step of the particle in slot `slot` of the arrays of p (which
 // start at most 2^30 slots before it)
 template <class S, class FS>
 __device__ __forceinline__ void step_particle32(const F32Args &p, S &sk, const FS &fs, unsigned slot) {
-  step_state32(p, sk, fs, load_state(p, slot * 4u), slot * 4u);
+  POut o;
+  if (step_state32(p, sk, fs, load_state(p, slot * 4u), o)) store_state(p, o, slot * 4u);
 }
 
 // the arguments with the particle arrays starting at slot `base` (uniform over
@@ -637,10 +672,20 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
   const F32Args q = from_slot(p, base);
   const unsigned lim = (unsigned)min(n_cap - base, kTile);
   const unsigned off0 = (unsigned)(chunk * kChunk + j0 * 64 + lane);
+  // software pipeline: the state of the lane's next particle is loaded before
+  // the current one is pushed; all its loads are waited for before the
+  // current particle's stores, so the wait never includes stores in flight
+  PState nx{kDead, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (off0 < lim) nx = load_state_next(q, off0 * 4u);
   for (int j = 0; j < PPT; j++) {
-    unsigned n = off0 + (unsigned)j * 64u;
-    asm("" : "+v"(n));  // keeps the slot a 32-bit offset from the scalar bases (no 64-bit pointer per array)
-    if (n < lim) step_particle32(q, sk, fs, n);
+    const PState cur = nx;
+    unsigned n = off0 + (unsigned)j * 64u, n1 = n + 64u;
+    asm("" : "+v"(n), "+v"(n1));  // keeps the slots 32-bit offsets from the scalar bases (no 64-bit pointer per array)
+    if (j + 1 < PPT && n1 < lim) nx = load_state_next(q, n1 * 4u);
+    POut o;
+    const bool st = n < lim && step_state32(q, sk, fs, cur, o);
+    wait_loads();
+    if (st) store_state(q, o, n * 4u);
   }
   sk.flush();
   __syncthreads();

@@ -170,48 +170,87 @@ __device__ __forceinline__ uint64_t wave_runs(uint32_t key, bool valid, int &hea
   return heads;
 }
 
-// runs per block of 256 keys (in logical order: logical index s is in slot
-// phys_of(s))
-template <class K>
-__global__ void __launch_bounds__(256) k_run_count(K keys, long long n, uint32_t *__restrict__ block_runs) {
-  __shared__ uint32_t wc[4];
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-  const bool valid = s < n;
-  int head, len;
-  const uint64_t heads = wave_runs(valid ? keys(phys_of(s)) : 0u, valid, head, len);
-  if (__lane_id() == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(heads);
-  __syncthreads();
-  if (threadIdx.x == 0) block_runs[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
-}
+// The run kernels take one chunk per block.  The chunk's keys are evaluated
+// in slot order (coalesced) and transposed to logical order in LDS (rows
+// padded to 17 words, so that neither direction has bank conflicts); the
+// block then walks them in groups of 256 logical keys, one record count
+// (block_runs / block_base entry) per group.
+constexpr int kGroups = kChunk / 256;
+constexpr int kRow = kPPT + 1;
+constexpr int kChunkLds = 64 * kRow;
 
-// the record index of this lane's run (block_base: exclusive scan of the
-// block counts)
-__device__ __forceinline__ uint32_t run_index(uint64_t heads, int head, const uint32_t *block_base) {
-  __shared__ uint32_t wc[4];
-  const int wave = threadIdx.x >> 6;
-  if (__lane_id() == 0) wc[wave] = (uint32_t)__popcll(heads);
+template <class K>
+__device__ __forceinline__ void chunk_keys(const K &keys, long long c0, uint32_t *t) {
+  for (int m = 0; m < kGroups; m++) {
+    const int w = m * 256 + (int)threadIdx.x;  // slot offset: iteration w / 64 of lane w % 64
+    t[(w % 64) * kRow + w / 64] = keys(c0 + w);
+  }
   __syncthreads();
-  uint32_t r = block_base[blockIdx.x];
-  for (int w = 0; w < wave; w++) r += wc[w];
-  const uint64_t below = head == 0 ? 0ull : (heads & (~0ull >> (64 - head)));
+}
+// logical offset s: lane s / kPPT, iteration s % kPPT
+__device__ __forceinline__ uint32_t chunk_key(const uint32_t *t, int s) { return t[(s / kPPT) * kRow + s % kPPT]; }
+
+// the runs of the block's kGroups groups of 256 logical keys: per lane the
+// key and its run (head lane, length) in each group, and per group and wave
+// the number of runs in wc[group * 4 + wave] (LDS, read after a barrier)
+struct ChunkRuns {
+  uint32_t key[kGroups];
+  uint64_t heads[kGroups];
+  int head[kGroups], len[kGroups];
+};
+__device__ __forceinline__ void chunk_runs(const uint32_t *t, ChunkRuns &cr, uint32_t *wc) {
+#pragma unroll
+  for (int m = 0; m < kGroups; m++) {
+    cr.key[m] = chunk_key(t, m * 256 + (int)threadIdx.x);
+    cr.heads[m] = wave_runs(cr.key[m], true, cr.head[m], cr.len[m]);
+    if (__lane_id() == 0) wc[m * 4 + (threadIdx.x >> 6)] = (uint32_t)__popcll(cr.heads[m]);
+  }
+  __syncthreads();
+}
+// the record index of this lane's run in group m (base: exclusive scan of the
+// group counts at this group)
+__device__ __forceinline__ uint32_t run_index(const ChunkRuns &cr, int m, uint32_t base, const uint32_t *wc) {
+  const int wave = threadIdx.x >> 6;
+  uint32_t r = base;
+  for (int w = 0; w < wave; w++) r += wc[m * 4 + w];
+  const int head = cr.head[m];
+  const uint64_t below = head == 0 ? 0ull : (cr.heads[m] & (~0ull >> (64 - head)));
   return r + (uint32_t)__popcll(below);
 }
 
+// runs per group of 256 logical keys
 template <class K>
-__global__ void __launch_bounds__(256) k_run_emit(K keys, long long n, const uint32_t *__restrict__ block_base,
+__global__ void __launch_bounds__(256) k_run_count(K keys, uint32_t *__restrict__ block_runs) {
+  __shared__ uint32_t t[kChunkLds];
+  __shared__ uint32_t wc[kGroups * 4];
+  chunk_keys(keys, (long long)blockIdx.x * kChunk, t);
+  ChunkRuns cr;
+  chunk_runs(t, cr, wc);
+  if (threadIdx.x < kGroups) {
+    const uint32_t *g = wc + threadIdx.x * 4;
+    block_runs[(long long)blockIdx.x * kGroups + threadIdx.x] = g[0] + g[1] + g[2] + g[3];
+  }
+}
+
+// one record (key, index, length) per run
+template <class K>
+__global__ void __launch_bounds__(256) k_run_emit(K keys, const uint32_t *__restrict__ block_base,
                                                   const uint32_t *count, long long cap, uint32_t *__restrict__ rkey,
                                                   uint32_t *__restrict__ ridx, uint32_t *__restrict__ rlen) {
   if (*count > cap) return;  // more runs than record slots: this sort is skipped
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-  const bool valid = s < n;
-  const uint32_t key = valid ? keys(phys_of(s)) : 0u;
-  int head, len;
-  const uint64_t heads = wave_runs(key, valid, head, len);
-  const uint32_t r = run_index(heads, head, block_base);
-  if (valid && head == (int)__lane_id()) {
-    rkey[r] = key;
-    ridx[r] = r;
-    rlen[r] = (uint32_t)len;
+  __shared__ uint32_t t[kChunkLds];
+  __shared__ uint32_t wc[kGroups * 4];
+  chunk_keys(keys, (long long)blockIdx.x * kChunk, t);
+  ChunkRuns cr;
+  chunk_runs(t, cr, wc);
+#pragma unroll
+  for (int m = 0; m < kGroups; m++) {
+    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kGroups + m], wc);
+    if (cr.head[m] == (int)__lane_id()) {
+      rkey[r] = cr.key[m];
+      ridx[r] = r;
+      rlen[r] = (uint32_t)cr.len[m];
+    }
   }
 }
 
@@ -230,45 +269,83 @@ __global__ void k_run_first(const uint32_t *__restrict__ order, const uint32_t *
   if (t < m) rfirst[order[t]] = lens[t];
 }
 
-// src[slot of the particle's rank] = its slot
-// (a skipped sort places every particle in its own slot)
+// src[rank] = the slot of the particle of that rank, indexed by logical
+// position (consecutive ranks are written by consecutive lanes); a skipped
+// sort leaves every particle in its own slot
 template <class K>
-__global__ void __launch_bounds__(256) k_run_place(K keys, long long n, const uint32_t *__restrict__ block_base,
+__global__ void __launch_bounds__(256) k_run_place(K keys, const uint32_t *__restrict__ block_base,
                                                    const uint32_t *count, long long cap,
                                                    const uint32_t *__restrict__ rfirst, uint32_t *__restrict__ src) {
-  const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
-  const bool valid = s < n;
+  const long long c0 = (long long)blockIdx.x * kChunk;
   if (*count > cap) {
-    if (valid) src[phys_of(s)] = (uint32_t)phys_of(s);
+    for (int s = threadIdx.x; s < kChunk; s += 256) src[c0 + s] = (uint32_t)phys_of(c0 + s);
     return;
   }
-  int head, len;
-  const uint64_t heads = wave_runs(valid ? keys(phys_of(s)) : 0u, valid, head, len);
-  const uint32_t r = run_index(heads, head, block_base);
+  __shared__ uint32_t t[kChunkLds];
+  __shared__ uint32_t wc[kGroups * 4];
+  chunk_keys(keys, c0, t);
+  ChunkRuns cr;
+  chunk_runs(t, cr, wc);
   const unsigned lane = __lane_id();
-  const uint32_t first = valid && head == (int)lane ? rfirst[r] : 0u;
-  const uint32_t rank = __shfl(first, head) + (lane - (unsigned)head);
-  if (valid) src[phys_of((long long)rank)] = (uint32_t)phys_of(s);
+#pragma unroll
+  for (int m = 0; m < kGroups; m++) {
+    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kGroups + m], wc);
+    const int head = cr.head[m];
+    const uint32_t first = head == (int)lane ? rfirst[r] : 0u;
+    const uint32_t rank = __shfl(first, head) + (lane - (unsigned)head);
+    src[rank] = (uint32_t)phys_of(c0 + m * 256 + (int)threadIdx.x);
+  }
 }
 
-// every particle array in one pass: out[p] = in[src[p]]
+// every particle array in one pass, one chunk per block: slot p of the
+// output takes the input slot src[logical position of p].  The block reads
+// its chunk of src coalesced and transposes it to slot order in LDS (rows
+// padded to 17 words); each thread then moves kGatherPer particles (256
+// slots apart) so that all their loads are in flight together (non-temporal
+// loads were slower: the slots of neighbouring particles share cache lines).
 struct Arrays7 {
   void *a[7];
 };
-__global__ void k_gather_f32(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src, long long n) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t q = src[p];
+constexpr int kGatherPer = kChunk / 256;
+template <class T>
+__device__ __forceinline__ void gather_array(const void *in, void *out, const uint32_t (&q)[kGatherPer],
+                                             long long p0) {
+  T x[kGatherPer];
 #pragma unroll
-  for (int k = 0; k < 7; k++) static_cast<uint32_t *>(out.a[k])[p] = static_cast<const uint32_t *>(in.a[k])[q];
+  for (int u = 0; u < kGatherPer; u++) x[u] = static_cast<const T *>(in)[q[u]];
+#pragma unroll
+  for (int u = 0; u < kGatherPer; u++) static_cast<T *>(out)[p0 + u * 256] = x[u];
 }
-__global__ void k_gather_f64(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src, long long n) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t q = src[p];
+// the source slots of this thread's kGatherPer output slots c0 + u*256 + tid
+__device__ __forceinline__ void gather_sources(const uint32_t *__restrict__ src, long long c0,
+                                               uint32_t (&q)[kGatherPer]) {
+  __shared__ uint32_t t[kChunkLds];
 #pragma unroll
-  for (int k = 0; k < 6; k++) static_cast<double *>(out.a[k])[p] = static_cast<const double *>(in.a[k])[q];
-  static_cast<uint8_t *>(out.a[6])[p] = static_cast<const uint8_t *>(in.a[6])[q];
+  for (int u = 0; u < kGatherPer; u++) {
+    const int s = u * 256 + (int)threadIdx.x;  // logical offset
+    t[(s / kPPT) * kRow + s % kPPT] = src[c0 + s];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kGatherPer; u++) {
+    const int w = u * 256 + (int)threadIdx.x;  // slot offset: lane w % 64, iteration w / 64
+    q[u] = t[(w % 64) * kRow + w / 64];
+  }
+}
+__global__ void __launch_bounds__(256) k_gather_f32(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src) {
+  const long long c0 = (long long)blockIdx.x * kChunk;
+  uint32_t q[kGatherPer];
+  gather_sources(src, c0, q);
+#pragma unroll
+  for (int k = 0; k < 7; k++) gather_array<uint32_t>(in.a[k], out.a[k], q, c0 + threadIdx.x);
+}
+__global__ void __launch_bounds__(256) k_gather_f64(Arrays7 in, Arrays7 out, const uint32_t *__restrict__ src) {
+  const long long c0 = (long long)blockIdx.x * kChunk;
+  uint32_t q[kGatherPer];
+  gather_sources(src, c0, q);
+#pragma unroll
+  for (int k = 0; k < 6; k++) gather_array<double>(in.a[k], out.a[k], q, c0 + threadIdx.x);
+  gather_array<uint8_t>(in.a[6], out.a[6], q, c0 + threadIdx.x);
 }
 
 // the sort key of the particle in slot j, from its state
@@ -348,6 +425,7 @@ bool ensure_alt(void *&alt, size_t bytes) {
 int sort_species(Ctx &c, Species &sp, bool exact) {
   const long long n = sp.n_cap;
   if (n <= 1 || !sp.chunked) return 0;
+  if (n % kChunk != 0) return -1;  // a chunked species fills whole chunks
   if (c.precision == 1 && !sp.f32.cell) return 0;
   const uint32_t ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
   const uint32_t dead_key = ncell;
@@ -374,8 +452,8 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
   if (4 * last > 3 * cap) cap = std::min(n, 2 * last + 65536);
   sp.rec_cap = cap;
   if (exact) cap = n;  // room for one run per particle: this sort cannot be skipped
-  // workspace (u32 words): src (n); block counts (nb + 1); scratch for the
-  // scans of the block counts and run lengths and for the records' digit
+  // workspace (u32 words): src (n); group counts (nb + 1); scratch for the
+  // scans of the group counts and run lengths and for the records' digit
   // histograms with their scan; the records (5 per slot)
   const long long rec_hist = (long long)kBins * ((cap + kPerBlock - 1) / kPerBlock);
   const long long scratch =
@@ -391,23 +469,24 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
   uint32_t *src = (uint32_t *)c.sort_ws, *block = src + n, *tmp = block + nb + 1;
   uint32_t *rk = tmp + scratch, *ri = rk + cap, *rk2 = ri + cap, *ri2 = rk2 + cap, *rlen = ri2 + cap;
   const uint32_t *count = block + nb;  // the number of runs, after the scan
-  const unsigned eb = (unsigned)((n + 255) / 256);
   const KeysF32 k32{c.g, (const uint32_t *)sp.f32.cell, dead_key};
   const KeysF64 k64{c.g, sp.f64.r, sp.f64.z, sp.f64.alive, dead_key};
   const bool f32 = c.precision == 1;
-  // runs per block and their scan (block[nb] = the number of runs)
+  // runs per group of 256 logical keys and their scan (block[nb] = the
+  // number of runs); n is a whole number of chunks, one per block
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, c.stream, block + nb, 1LL, 0u);
+  const unsigned chunks = (unsigned)(n / kChunk);
   if (f32)
-    hipLaunchKernelGGL(k_run_count<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block);
+    hipLaunchKernelGGL(k_run_count<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block);
   else
-    hipLaunchKernelGGL(k_run_count<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block);
+    hipLaunchKernelGGL(k_run_count<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block);
   exclusive_scan(block, nb + 1, tmp, c.stream);
   if (f32)
-    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, count, cap, rk,
-                       ri, rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block, count, cap, rk, ri,
+                       rlen);
   else
-    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, count, cap, rk,
-                       ri, rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block, count, cap, rk, ri,
+                       rlen);
   // records by key (stable), then each run's first rank
   radix_pairs(rk, ri, rk2, ri2, count, cap, bits, tmp, tmp + rec_hist, c.stream);
   const unsigned mb = (unsigned)((cap + 255) / 256);
@@ -416,11 +495,9 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
   exclusive_scan(lens, cap, tmp, c.stream);
   hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, c.stream, ri, lens, count, cap, rlen);
   if (f32)
-    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3((unsigned)nb), dim3(256), 0, c.stream, k32, n, block, count, cap,
-                       rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block, count, cap, rlen, src);
   else
-    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3((unsigned)nb), dim3(256), 0, c.stream, k64, n, block, count, cap,
-                       rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block, count, cap, rlen, src);
   // the run count for the next sort's capacity
   if (hipMemcpyAsync(sp.runs_host, count, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
       hipEventRecord(sp.runs_ev, c.stream) != hipSuccess)
@@ -438,7 +515,7 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
       in.a[k] = *ap[k];
       out.a[k] = *bp[k];
     }
-    hipLaunchKernelGGL(k_gather_f32, dim3(eb), dim3(256), 0, c.stream, in, out, src, n);
+    hipLaunchKernelGGL(k_gather_f32, dim3(chunks), dim3(256), 0, c.stream, in, out, src);
     for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
   } else {
     SpeciesF64 &a = sp.f64, &b = sp.f64_alt;
@@ -451,7 +528,7 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
       in.a[k] = *ap[k];
       out.a[k] = *bp[k];
     }
-    hipLaunchKernelGGL(k_gather_f64, dim3(eb), dim3(256), 0, c.stream, in, out, src, n);
+    hipLaunchKernelGGL(k_gather_f64, dim3(chunks), dim3(256), 0, c.stream, in, out, src);
     for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
   }
   return 0;
