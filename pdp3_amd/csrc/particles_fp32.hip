@@ -150,25 +150,28 @@ struct GlobalFields {
 // rows bi..bi+kFR-1, columns bk..bk+kFK-1 of the six components, anchored one
 // row and one column below the block's first cell (ai, ak): a particle in cell
 // (i, k) reads rows i-1..i+1 and columns k-1..k+1, so every cell with
-// i in [ai, ai+1] and k in [ak, ak+kFK-3] is served from the box
+// i in [ai, ai+1] and k in [ak, ak+kFK-3] is served from the box.  Each entry
+// is a pair (node k, node k+1) of its row, so that the four nodes of one
+// component's stencil are two 8-byte entries kFK apart (one LDS instruction)
+// and the six components differ only by a constant offset.
 constexpr int kFR = 4, kFK = 64, kFComp = 6, kFSize = kFComp * kFR * kFK;
-// the LDS array: the field box, then two float4 row entries for each box row
-constexpr int kFBoxFloats = kFSize + 8 * kFR;
+// the LDS array (floats): the field box (two floats per entry), then two
+// float4 row entries for each box row
+constexpr int kFBoxFloats = 2 * kFSize + 8 * kFR;
 
-__device__ __forceinline__ float gather_box(const float *F, RadW rw, AxW aw, int bi, int bk) {
-  const float *row0 = F + (rw.ir - bi) * kFK + (aw.kz - bk);
-  float f00 = row0[0], f01 = row0[1], f10 = row0[kFK], f11 = row0[kFK + 1];
+__device__ __forceinline__ float gather_box(const float2 *e, RadW rw, AxW aw) {
+  const float2 lo = e[0], hi = e[kFK];
   float b = aw.b, ob = 1.f - aw.b;
-  return (f00 * ob + f01 * b) * rw.A + (f10 * ob + f11 * b) * rw.B;
+  return (lo.x * ob + lo.y * b) * rw.A + (hi.x * ob + hi.y * b) * rw.B;
 }
 
 struct BoxFields {
-  const float *box;  // LDS, kFComp x kFR x kFK, then the row entries of rows ai-1 .. ai+2
+  const float *box;  // LDS, kFComp x kFR x kFK pairs, then the row entries of rows ai-1 .. ai+2
   int ai, ak;        // first cell of the block
   __device__ __forceinline__ void row(const F32Args &p, int i, float4 &t0, float4 &t1) const {
     const int li = i - ai + 1;
     if ((unsigned)li < (unsigned)kFR) {
-      const float4 *t = reinterpret_cast<const float4 *>(box + kFSize);
+      const float4 *t = reinterpret_cast<const float4 *>(box + 2 * kFSize);
       t0 = t[2 * li];
       t1 = t[2 * li + 1];
     } else {
@@ -182,12 +185,16 @@ struct BoxFields {
     if (inbox) {
       const int bi = ai - 1, bk = ak - 1;
       constexpr int C = kFR * kFK;
-      const float e0 = gather_box(box + 0 * C, rl, zo, bi, bk);
-      const float e1 = gather_box(box + 1 * C, ro, zo, bi, bk);
-      const float e2 = gather_box(box + 2 * C, ro, zh, bi, bk);
-      const float h0 = gather_box(box + 3 * C, ro, zh, bi, bk);
-      const float h1 = gather_box(box + 4 * C, rl, zh, bi, bk);
-      const float h2 = gather_box(box + 5 * C, rl, zo, bi, bk);
+      const float2 *f = reinterpret_cast<const float2 *>(box);
+      // the stencil's first entry for each (radial, axial) weight pair
+      const int rlo = (rl.ir - bi) * kFK, roo = (ro.ir - bi) * kFK, zoo = zo.kz - bk, zho = zh.kz - bk;
+      const float2 *lo_o = f + (rlo + zoo), *oo_o = f + (roo + zoo), *oo_h = f + (roo + zho), *lo_h = f + (rlo + zho);
+      const float e0 = gather_box(lo_o + 0 * C, rl, zo);
+      const float e1 = gather_box(oo_o + 1 * C, ro, zo);
+      const float e2 = gather_box(oo_h + 2 * C, ro, zh);
+      const float h0 = gather_box(oo_h + 3 * C, ro, zh);
+      const float h1 = gather_box(lo_h + 4 * C, rl, zh);
+      const float h2 = gather_box(lo_o + 5 * C, rl, zo);
       e[0] = e0; e[1] = e1; e[2] = e2;
       h[0] = h0; h[1] = h1; h[2] = h2;
     } else {
@@ -204,24 +211,29 @@ struct BoxFields {
 };
 
 // block prologue: fill the field box (nodes outside the grid read as 0; no
-// particle that the box serves reads them)
+// particle that the box serves reads them, nor the second half of a row's
+// last entry)
 __device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int ai, int ak) {
   const int bi = ai - 1, bk = ak - 1;
   if (threadIdx.x < 2 * kFR) {
     const int r = min(max(bi + (int)(threadIdx.x >> 1), 0), p.nr - 1);
-    reinterpret_cast<float4 *>(box + kFSize)[threadIdx.x] = p.rows[2 * r + (threadIdx.x & 1)];
+    reinterpret_cast<float4 *>(box + 2 * kFSize)[threadIdx.x] = p.rows[2 * r + (threadIdx.x & 1)];
   }
   const double *src[kFComp] = {p.er, p.ephi, p.ez, p.hhr, p.hhphi, p.hhz};
+  float2 *f = reinterpret_cast<float2 *>(box);
 #pragma unroll
   for (int comp = 0; comp < kFComp; comp++) {
     const double *F = src[comp];
     const float sc = comp < 3 ? p.escale : p.bscale;
+    // one wavefront per box row (kFK = 64 columns): each node is loaded once
+    // and the next column's value comes from the next lane
     for (int t = threadIdx.x; t < kFR * kFK; t += blockDim.x) {
       const int r = t / kFK, col = t - r * kFK;
       const int gi = bi + r, gk = bk + col;
-      float v = 0.f;
-      if ((unsigned)gi < (unsigned)p.nr && (unsigned)gk < (unsigned)p.nz) v = (float)F[(size_t)gi * p.nz + gk] * sc;
-      box[comp * kFR * kFK + t] = v;
+      float v0 = 0.f;
+      if ((unsigned)gi < (unsigned)p.nr && (unsigned)gk < (unsigned)p.nz) v0 = (float)F[(size_t)gi * p.nz + gk] * sc;
+      const float v1 = __shfl_down(v0, 1);  // lane 63 (the row's last entry) gets its own value: never read
+      f[comp * kFR * kFK + t] = make_float2(v0, v1);
     }
   }
 }
@@ -651,6 +663,7 @@ __device__ __forceinline__ F32Args from_slot(const F32Args &p, long long base) {
 template <int PPT, int WAVES>
 __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink &gs, long long base, long long n_cap,
                                              double *box, float *fbox) {
+  static_assert(kFK == 64, "one wavefront per field box row");
   static_assert(kPPT % PPT == 0 && (WAVES * PPT) % kPPT == 0, "tile shape");
   int ai = 0, ak = 0;
   if (base < n_cap) {
