@@ -42,10 +42,14 @@ __device__ __forceinline__ D cst(double x) {
   for (int j = 0; j < kDN; j++) r.d[j] = 0.0;
   return r;
 }
+// x as input `input` of pass `pass` (derivative 1 in its direction); selects
+// on constant indices, so that no dual number is addressed through memory
 __device__ __forceinline__ D seed(double x, int input, int pass) {
-  D r = cst(x);
-  const int j = input - pass * kDN;
-  if (j >= 0 && j < kDN) r.d[j] = 1.0;
+  D r;
+  r.v = x;
+  const int jin = input - pass * kDN;
+#pragma unroll
+  for (int j = 0; j < kDN; j++) r.d[j] = j == jin ? 1.0 : 0.0;
   return r;
 }
 __device__ __forceinline__ D operator+(D a, D b) {
@@ -532,12 +536,13 @@ __device__ void deposit_j_d(const GridDesc &g, AdjSink &acc, D q, D pr, D pz, D 
     strict_d(g, acc, q, pr, pz, por, poz);
     return;
   }
+  if (res_cell == 0) {  // the common case, without the segment list (which lives in scratch memory)
+    segment_d(g, acc, q, pr, pz, por, poz, i_n, k_n);
+    return;
+  }
   SegD s[3];
   int ns = 0;
-  s[0] = SegD{pr, pz, por, poz, i_n, k_n};
-  if (res_cell == 0) {
-    ns = 1;
-  } else if (res_cell == 1) {
+  if (res_cell == 1) {
     if ((i_n != i_o) && (k_n == k_o)) {
       if (por.v > (i_n + 1) * dr) {
         D a = (por - pr) / (poz - pz);
@@ -687,7 +692,7 @@ __global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, 
   }
   const double r0v = s.r[i], z0v = s.z[i];
   const double lam_r = s.lr[i], lam_z = s.lz[i], lam_v0 = s.lvr[i], lam_v1 = s.lvphi[i], lam_v2 = s.lvz[i];
-  double lin[kInputs];
+  double lin[kInputs] = {};
   for (int pass = 0; pass < kPasses; pass++) {
     AdjSink sk;
     sk.g = &g;
@@ -737,7 +742,8 @@ __global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, 
       for (int j = 0; j < kDN; j++)
         sk.acc[j] += lam_r * r.d[j] + lam_z * z.d[j] + lam_v0 * v[0].d[j] + lam_v1 * v[1].d[j] + lam_v2 * v[2].d[j];
     }
-    for (int j = 0; j < kDN; j++) lin[pass * kDN + j] = sk.acc[j];
+#pragma unroll
+    for (int o = 0; o < kInputs; o++) lin[o] = o / kDN == pass ? sk.acc[o % kDN] : lin[o];
   }
   s.lr[i] = lin[0];
   s.lz[i] = lin[1];
