@@ -22,6 +22,7 @@
 // particles_fp64.hip expression for expression.
 #include "context.hpp"
 #include "crmath.hpp"
+#include "sink.hpp"
 
 namespace pdp3 {
 
@@ -207,16 +208,36 @@ __device__ __forceinline__ D gather_d(const GridDesc &g, const double *F, D r, D
   return gw.w[0] * row0[0] + gw.w[1] * row1[0] + gw.w[2] * row0[1] + gw.w[3] * row1[1];
 }
 
-__device__ __forceinline__ void scatter_gather(const GridDesc &g, double *LF, double r, double z, bool lo, bool zhalf,
-                                               double lam) {
+// the adjoints of the gathered fields of one block's particles are summed in
+// an LDS box of kABR x kABK nodes per component (anchored one row and two
+// columns below the block's smallest cell) and added to the grid once per
+// block; stencils outside the box go to the grid directly
+constexpr int kABR = 4, kABK = 64, kABComp = 6;
+struct AdjBox {
+  double *box;  // LDS, kABComp x kABR x kABK
+  int bi, bk;   // node of box[.][0][0]
+};
+__device__ __forceinline__ void box_scatter(const GridDesc &g, const AdjBox &b, int comp, double *LF, double r,
+                                            double z, bool lo, bool zhalf, double lam) {
   if (lam == 0.0) return;
   GatherW gw = gather_w(g, cst(r), cst(z), lo, zhalf);
-  double *row0 = LF + (size_t)gw.ir * (size_t)g.nz + gw.kz;
-  double *row1 = row0 + g.nz;
-  unsafeAtomicAdd(row0, gw.w[0].v * lam);
-  unsafeAtomicAdd(row1, gw.w[1].v * lam);
-  unsafeAtomicAdd(row0 + 1, gw.w[2].v * lam);
-  unsafeAtomicAdd(row1 + 1, gw.w[3].v * lam);
+  // weights 0..3 multiply the nodes (ir, kz), (ir+1, kz), (ir, kz+1), (ir+1, kz+1)
+  const double v00 = gw.w[0].v * lam, v10 = gw.w[1].v * lam, v01 = gw.w[2].v * lam, v11 = gw.w[3].v * lam;
+  const int li = gw.ir - b.bi, lk = gw.kz - b.bk;
+  if ((unsigned)li < (unsigned)(kABR - 1) && (unsigned)lk < (unsigned)(kABK - 1)) {
+    double *q = b.box + (comp * kABR + li) * kABK + lk;
+    atomicAdd(q, v00);
+    atomicAdd(q + kABK, v10);
+    atomicAdd(q + 1, v01);
+    atomicAdd(q + kABK + 1, v11);
+  } else {
+    double *row0 = LF + (size_t)gw.ir * (size_t)g.nz + gw.kz;
+    double *row1 = row0 + g.nz;
+    unsafeAtomicAdd(row0, v00);
+    unsafeAtomicAdd(row1, v10);
+    unsafeAtomicAdd(row0 + 1, v01);
+    unsafeAtomicAdd(row1 + 1, v11);
+  }
 }
 
 // the deposit "sink" of the adjoint: every contribution v at node (i, k) of
@@ -682,14 +703,9 @@ __device__ __forceinline__ void walls_d(const GridDesc &g, bool is_bunch, D &x, 
 }
 
 // vector-Jacobian product of one particle's step
-__global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, AdjGrid lg, AdjSpecies s,
-                                                       int pusher) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.n) return;
-  if (s.alive[i] != 1) {
-    s.lr[i] = s.lz[i] = s.lvr[i] = s.lvphi[i] = s.lvz[i] = 0.0;
-    return;
-  }
+// the vector-Jacobian product of one living particle's step
+__device__ __forceinline__ void adj_particle(const GridDesc &g, const AdjFields &f, const AdjGrid &lg,
+                                             const AdjSpecies &s, int pusher, long long i, const AdjBox &b) {
   const double r0v = s.r[i], z0v = s.z[i];
   const double lam_r = s.lr[i], lam_z = s.lz[i], lam_v0 = s.lvr[i], lam_v1 = s.lvphi[i], lam_v2 = s.lvz[i];
   double lin[kInputs] = {};
@@ -751,12 +767,43 @@ __global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, 
   s.lvphi[i] = lin[3];
   s.lvz[i] = lin[4];
   s.lw[i] += lin[11];
-  scatter_gather(g, lg.ler, r0v, z0v, true, false, lin[5]);
-  scatter_gather(g, lg.lephi, r0v, z0v, false, false, lin[6]);
-  scatter_gather(g, lg.lez, r0v, z0v, false, true, lin[7]);
-  scatter_gather(g, lg.lhr, r0v, z0v, false, true, lin[8]);
-  scatter_gather(g, lg.lhphi, r0v, z0v, true, true, lin[9]);
-  scatter_gather(g, lg.lhz, r0v, z0v, true, false, lin[10]);
+  box_scatter(g, b, 0, lg.ler, r0v, z0v, true, false, lin[5]);
+  box_scatter(g, b, 1, lg.lephi, r0v, z0v, false, false, lin[6]);
+  box_scatter(g, b, 2, lg.lez, r0v, z0v, false, true, lin[7]);
+  box_scatter(g, b, 3, lg.lhr, r0v, z0v, false, true, lin[8]);
+  box_scatter(g, b, 4, lg.lhphi, r0v, z0v, true, true, lin[9]);
+  box_scatter(g, b, 5, lg.lhz, r0v, z0v, true, false, lin[10]);
+}
+
+__global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, AdjGrid lg, AdjSpecies s,
+                                                       int pusher) {
+  __shared__ double box[kABComp * kABR * kABK];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < s.n && s.alive[i] == 1;
+  // the box anchor: the smallest cell of the block's living particles
+  int ci = INT_MAX, ck = INT_MAX;
+  if (live) {
+    ci = max(cell_number(s.r[i], g.dr), 0);
+    ck = max(cell_number(s.z[i], g.dz), 0);
+  }
+  block_min2(ci, ck);
+  if (ci == INT_MAX) ci = ck = 0;
+  for (int t = threadIdx.x; t < kABComp * kABR * kABK; t += blockDim.x) box[t] = 0.0;
+  __syncthreads();
+  const AdjBox b{box, ci - 1, ck - 2};
+  if (live) adj_particle(g, f, lg, s, pusher, i, b);
+  else if (i < s.n) s.lr[i] = s.lz[i] = s.lvr[i] = s.lvphi[i] = s.lvz[i] = 0.0;
+  __syncthreads();
+  double *const lf[kABComp] = {lg.ler, lg.lephi, lg.lez, lg.lhr, lg.lhphi, lg.lhz};
+  for (int t = threadIdx.x; t < kABComp * kABR * kABK; t += blockDim.x) {
+    const double v = box[t];
+    if (v == 0.0) continue;
+    const int comp = t / (kABR * kABK), rem = t - comp * kABR * kABK;
+    const int gi = b.bi + rem / kABK, gk = b.bk + rem % kABK;
+    // nodes off the grid never receive a stencil (gather_w clamps its cell)
+    if ((unsigned)gi < (unsigned)g.nr && (unsigned)gk < (unsigned)g.nz)
+      unsafeAtomicAdd(lf[comp] + (size_t)gi * (size_t)g.nz + gk, v);
+  }
 }
 
 // ------------------------------------------------------------- the fields
