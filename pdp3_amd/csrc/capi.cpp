@@ -849,11 +849,25 @@ int pdp3_bunch_inject(pdp3_ctx *ctx, int index) {
 namespace {
 int sort_plasma(pdp3_ctx *ctx, bool exact);
 
+// the periodic sorts' buffers, allocated on the first step that sorts
+// periodically (not inside the sort itself, so that a warmed-up loop
+// allocates nothing)
+int prepare_sorts(pdp3_ctx *ctx) {
+  Ctx &c = ctx->c;
+  for (int j : c.order) {
+    Species &sp = c.species[j];
+    if (sp.kind != PDP3_SPECIES_PLASMA || sp.sort_prepared) continue;
+    if (sort_prepare(c, sp)) return fail(ctx, PDP3_ERR_OOM, "sort buffers");
+  }
+  return PDP3_OK;
+}
+
 // one loop iteration (src/loadInitParam.cpp:471-587); after_h (may be null)
 // runs right after the H update, when E^n and H at the half time are final
 int step_one(pdp3_ctx *ctx, const std::function<int()> *after_h) {
   Ctx &c = ctx->c;
   int st;
+  if (c.sort_interval > 0 && (st = prepare_sorts(ctx))) return st;
   if (c.sort_interval > 0 && c.steps_done > 0 && c.steps_done % c.sort_interval == 0) {
     if ((st = sort_plasma(ctx, false))) return st;
   }
@@ -1371,6 +1385,9 @@ int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps) {
   if (!e || nsteps < 0) return PDP3_ERR_BAD_ARG;
   int st;
   bool timing = false;
+  if (e->sort_interval > 0)
+    for (pdp3_ctx *ctx : e->ctxs)
+      if ((st = prepare_sorts(ctx))) return efail(e, st, ctx->c.last_error);
   for (int s = 0; s < nsteps; s++) {
     if (e->sort_interval > 0 && e->steps_done > 0 && e->steps_done % e->sort_interval == 0) {
       ENS_HIP(e, hipStreamSynchronize(e->stream));

@@ -102,6 +102,7 @@ struct Species {
   long long rec_cap = 0;
   uint32_t *runs_host = nullptr;
   hipEvent_t runs_ev = nullptr;
+  bool sort_prepared = false;  // the buffers of periodic sorts are allocated
   // bunch
   double *inject_u = nullptr;
   double duration = 0, hole_duration = 0, velocity = 0;
@@ -221,5 +222,10 @@ void launch_alive_to_logical(Ctx &c, const uint8_t *phys, uint8_t *logical, long
 // run count, and a sort that finds more runs than that is skipped (the state
 // is left in its order) and sizes the next one
 int sort_species(Ctx &c, Species &sp, bool exact);
+// allocates what the periodic sorts of a plasma species use (the second
+// particle buffers, the workspace for the initial record space, the pinned
+// run count), so that a loop that sorts allocates nothing after its first
+// step; returns 0, or -3 when out of memory
+int sort_prepare(Ctx &c, Species &sp);
 
 }  // namespace pdp3

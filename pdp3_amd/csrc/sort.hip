@@ -420,52 +420,100 @@ bool ensure_alt(void *&alt, size_t bytes) {
   return true;
 }
 
+// workspace bytes for n slots and cap run records (u32 words): src (n); group
+// counts (nb + 1); scratch for the scans of the group counts and run lengths
+// and for the records' digit histograms with their scan; the records (5 per
+// slot)
+struct SortWs {
+  long long nb, rec_hist, scratch;
+  size_t bytes;
+};
+SortWs sort_ws(long long n, long long cap) {
+  SortWs w;
+  w.nb = (n + 255) / 256;
+  w.rec_hist = (long long)kBins * ((cap + kPerBlock - 1) / kPerBlock);
+  w.scratch = std::max(std::max(scan_scratch(w.nb + 1), scan_scratch(cap)), w.rec_hist + scan_scratch(w.rec_hist)) +
+              1024;
+  w.bytes = (size_t)(n + (w.nb + 1) + w.scratch + 5 * cap) * sizeof(uint32_t);
+  return w;
+}
+bool ensure_ws(Ctx &c, size_t need) {
+  if (c.sort_ws_bytes >= need) return true;
+  if (c.sort_ws) (void)hipFree(c.sort_ws);
+  c.sort_ws = nullptr;
+  c.sort_ws_bytes = 0;
+  if (hipMalloc(&c.sort_ws, need) != hipSuccess) return false;
+  c.sort_ws_bytes = need;
+  return true;
+}
+// the pinned run count and its event
+bool ensure_runs(Species &sp) {
+  if (sp.runs_host) return true;
+  if (hipHostMalloc((void **)&sp.runs_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    sp.runs_host = nullptr;
+    return false;
+  }
+  *sp.runs_host = 0;
+  if (hipEventCreateWithFlags(&sp.runs_ev, hipEventDisableTiming) != hipSuccess) {
+    sp.runs_ev = nullptr;
+    return false;
+  }
+  return true;
+}
+// the second buffer of every particle array
+bool ensure_alts(Ctx &c, Species &sp) {
+  const long long n = sp.n_cap;
+  if (c.precision == 1) {
+    SpeciesF32 &b = sp.f32_alt;
+    void **bp[7] = {(void **)&b.cell, (void **)&b.fr, (void **)&b.fz, (void **)&b.vr, (void **)&b.vphi,
+                    (void **)&b.vz, (void **)&b.w};
+    for (int k = 0; k < 7; k++)
+      if (!ensure_alt(*bp[k], (size_t)n * 4)) return false;
+  } else {
+    SpeciesF64 &b = sp.f64_alt;
+    void **bp[7] = {(void **)&b.r, (void **)&b.z, (void **)&b.vr, (void **)&b.vphi, (void **)&b.vz, (void **)&b.w,
+                    (void **)&b.alive};
+    for (int k = 0; k < 7; k++)
+      if (!ensure_alt(*bp[k], (size_t)n * (k < 6 ? 8 : 1))) return false;
+  }
+  return true;
+}
+bool sortable(const Ctx &c, const Species &sp) {
+  return sp.n_cap > 1 && sp.chunked && !(c.precision == 1 && !sp.f32.cell);
+}
+
 }  // namespace
+
+int sort_prepare(Ctx &c, Species &sp) {
+  if (sp.sort_prepared || !sortable(c, sp)) return 0;
+  const long long n = sp.n_cap;
+  if (sp.rec_cap <= 0) sp.rec_cap = std::min(n, n / 4 + 65536);
+  if (!ensure_runs(sp) || !ensure_ws(c, sort_ws(n, sp.rec_cap).bytes) || !ensure_alts(c, sp)) return -3;
+  sp.sort_prepared = true;
+  return 0;
+}
 
 int sort_species(Ctx &c, Species &sp, bool exact) {
   const long long n = sp.n_cap;
-  if (n <= 1 || !sp.chunked) return 0;
+  if (!sortable(c, sp)) return 0;
   if (n % kChunk != 0) return -1;  // a chunked species fills whole chunks
-  if (c.precision == 1 && !sp.f32.cell) return 0;
   const uint32_t ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
   const uint32_t dead_key = ncell;
   int bits = 1;
   while ((1ull << bits) <= (unsigned long long)dead_key + 1) bits++;
-  const long long nb = (n + 255) / 256;  // blocks of the run kernels
   // record capacity: grown from the previous sort's run count (read back
   // asynchronously), so that no sort waits for the device
-  if (!sp.runs_host) {
-    if (hipHostMalloc((void **)&sp.runs_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
-      sp.runs_host = nullptr;
-      return -3;
-    }
-    *sp.runs_host = 0;
-    if (hipEventCreateWithFlags(&sp.runs_ev, hipEventDisableTiming) != hipSuccess) {
-      sp.runs_ev = nullptr;
-      return -3;
-    }
-  } else if (hipEventSynchronize(sp.runs_ev) != hipSuccess) {
-    return -3;
-  }
+  const bool first = !sp.runs_host;
+  if (!ensure_runs(sp)) return -3;
+  if (!first && hipEventSynchronize(sp.runs_ev) != hipSuccess) return -3;
   const long long last = *sp.runs_host;
   long long cap = sp.rec_cap > 0 ? sp.rec_cap : std::min(n, n / 4 + 65536);
   if (4 * last > 3 * cap) cap = std::min(n, 2 * last + 65536);
   sp.rec_cap = cap;
   if (exact) cap = n;  // room for one run per particle: this sort cannot be skipped
-  // workspace (u32 words): src (n); group counts (nb + 1); scratch for the
-  // scans of the group counts and run lengths and for the records' digit
-  // histograms with their scan; the records (5 per slot)
-  const long long rec_hist = (long long)kBins * ((cap + kPerBlock - 1) / kPerBlock);
-  const long long scratch =
-      std::max(std::max(scan_scratch(nb + 1), scan_scratch(cap)), rec_hist + scan_scratch(rec_hist)) + 1024;
-  const size_t need = (size_t)(n + (nb + 1) + scratch + 5 * cap) * sizeof(uint32_t);
-  if (c.sort_ws_bytes < need) {
-    if (c.sort_ws) (void)hipFree(c.sort_ws);
-    c.sort_ws = nullptr;
-    c.sort_ws_bytes = 0;
-    if (hipMalloc(&c.sort_ws, need) != hipSuccess) return -3;
-    c.sort_ws_bytes = need;
-  }
+  const SortWs ws = sort_ws(n, cap);
+  const long long nb = ws.nb, rec_hist = ws.rec_hist, scratch = ws.scratch;
+  if (!ensure_ws(c, ws.bytes) || !ensure_alts(c, sp)) return -3;
   uint32_t *src = (uint32_t *)c.sort_ws, *block = src + n, *tmp = block + nb + 1;
   uint32_t *rk = tmp + scratch, *ri = rk + cap, *rk2 = ri + cap, *ri2 = rk2 + cap, *rlen = ri2 + cap;
   const uint32_t *count = block + nb;  // the number of runs, after the scan
@@ -511,7 +559,6 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
     void **bp[7] = {(void **)&b.cell, (void **)&b.fr, (void **)&b.fz, (void **)&b.vr, (void **)&b.vphi,
                     (void **)&b.vz, (void **)&b.w};
     for (int k = 0; k < 7; k++) {
-      if (!ensure_alt(*bp[k], (size_t)n * 4)) return -3;
       in.a[k] = *ap[k];
       out.a[k] = *bp[k];
     }
@@ -524,7 +571,6 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
     void **bp[7] = {(void **)&b.r, (void **)&b.z, (void **)&b.vr, (void **)&b.vphi, (void **)&b.vz, (void **)&b.w,
                     (void **)&b.alive};
     for (int k = 0; k < 7; k++) {
-      if (!ensure_alt(*bp[k], (size_t)n * (k < 6 ? 8 : 1))) return -3;
       in.a[k] = *ap[k];
       out.a[k] = *bp[k];
     }
