@@ -33,11 +33,11 @@ struct HPtrs {
 // tiles of R x C nodes, one thread per node; the block stages the E (for H)
 // or H (for E) values of its tile plus one halo row and column in LDS, one
 // barrier, then every thread reads its stencil from LDS and its own node's
-// values straight from HBM.  Measured at 512 x 8192 nodes: H on 2 x 256
-// tiles 0.069 ms (per-node kernel reading neighbours through the caches:
-// 0.073), E on 1 x 256 tiles 0.103 ms (per-node: 0.103); 4 x 64 and 2 x 128
-// tiles were slower for E, and loading the node's own values before the
-// barrier was slower for both.
+// values straight from HBM.  Measured (rocprofv3) at 513 x 8193 nodes: H on
+// 2 x 256 tiles 0.069 ms, E on 1 x 256 tiles 0.085 ms, against 0.070 and
+// 0.082 ms for per-node kernels reading the neighbours through the caches;
+// 4 x 64 and 2 x 128 tiles were slower for E, and loading the node's own
+// values before the barrier was slower for both.
 template <int R, int C>
 struct TileShape {
   static constexpr int kRows = R, kCols = C, kThreads = R * C, kStride = C + 1;
