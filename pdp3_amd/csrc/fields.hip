@@ -204,12 +204,47 @@ struct EnsGrid {
   double *jr, *jphi, *jz, *rho;
 };
 
+// the grid tables are read through mirrors whose pointers are in the global
+// address space (pointers read from device memory are otherwise generic, and
+// every access would be a flat instruction on a 64-bit per-lane address)
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+struct HPtrsG {
+  gptr<const double> er, ephi, ez;
+  gptr<double> hr, hphi, hz, hhr, hhphi, hhz;
+};
+struct EPtrsG {
+  gptr<const double> hr, hphi, hz, jr, jphi, jz, ke, kh;
+  gptr<double> er, ephi, ez;
+};
+struct EnsGridG {
+  GridDesc g;
+  HPtrsG h;
+  EPtrsG e;
+  gptr<double> jr, jphi, jz, rho;
+};
+static_assert(sizeof(EnsGridG) == sizeof(EnsGrid) && offsetof(EnsGridG, h) == offsetof(EnsGrid, h) &&
+                  offsetof(EnsGridG, e) == offsetof(EnsGrid, e) && offsetof(EnsGridG, jr) == offsetof(EnsGrid, jr) &&
+                  sizeof(HPtrsG) == sizeof(HPtrs) && sizeof(EPtrsG) == sizeof(EPtrs),
+              "EnsGridG mirrors EnsGrid");
+__device__ __forceinline__ HPtrs from_global(const HPtrsG &g) {
+  return HPtrs{(const double *)g.er,  (const double *)g.ephi, (const double *)g.ez,
+               (double *)g.hr,        (double *)g.hphi,       (double *)g.hz,
+               (double *)g.hhr,       (double *)g.hhphi,      (double *)g.hhz};
+}
+__device__ __forceinline__ EPtrs from_global(const EPtrsG &g) {
+  return EPtrs{(const double *)g.hr, (const double *)g.hphi, (const double *)g.hz, (const double *)g.jr,
+               (const double *)g.jphi, (const double *)g.jz, (const double *)g.ke, (const double *)g.kh,
+               (double *)g.er, (double *)g.ephi, (double *)g.ez};
+}
+
 __global__ void __launch_bounds__(HTile::kThreads) k_ens_h(const EnsGrid *__restrict__ grids,
                                                            const EnsRow *__restrict__ tiles, int n_tiles) {
   __shared__ double s[3 * HTile::kLds];
   for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
     const EnsRow r = tiles[it];
-    h_tile<HTile>(grids[r.inst].g, grids[r.inst].h, r.i0, r.k0, s);
+    const EnsGridG &G = reinterpret_cast<const EnsGridG *>(grids)[r.inst];
+    h_tile<HTile>(G.g, from_global(G.h), r.i0, r.k0, s);
     __syncthreads();  // the next tile rewrites the staged rows
   }
 }
@@ -219,14 +254,15 @@ __global__ void __launch_bounds__(ETile::kThreads) k_ens_e(const EnsGrid *__rest
   __shared__ double s[3 * ETile::kLds];
   for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
     const EnsRow r = tiles[it];
-    e_tile<ETile>(grids[r.inst].g, grids[r.inst].e, r.i0, r.k0, s);
+    const EnsGridG &G = reinterpret_cast<const EnsGridG *>(grids)[r.inst];
+    e_tile<ETile>(G.g, from_global(G.e), r.i0, r.k0, s);
     __syncthreads();  // the next tile rewrites the staged rows
   }
 }
 
 // Current::reset_j and the bunch density reset of every instance
 __global__ void __launch_bounds__(256) k_ens_zero(const EnsGrid *__restrict__ grids) {
-  const EnsGrid &G = grids[blockIdx.y];
+  const EnsGridG &G = reinterpret_cast<const EnsGridG *>(grids)[blockIdx.y];
   const size_t n = (size_t)G.g.nr * (size_t)G.g.nz;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     G.jr[p] = 0.0;

@@ -767,15 +767,96 @@ struct EnsItem {
   long long base, n_cap;
 };
 
+// A pointer read from device memory is generic (flat) to the compiler: its
+// loads, stores and atomics become flat instructions on 64-bit per-lane
+// addresses, which also hold up the kernel's LDS waits.  The ensemble's
+// argument tables are therefore read through mirrors of F32Args and
+// GlobalSink whose pointers are in the global address space (same layout);
+// converted to the generic structs, every access derived from them compiles
+// to global instructions on scalar bases, as for kernel arguments.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+struct F32ArgsG {
+  int nr, nz;
+  float hr, hz, inv_c2, c1, bscale, escale, kz, kr, kp;
+  int pusher, is_bunch, want_rho, n;
+  gptr<const double> er, ephi, ez, hhr, hhphi, hhz;
+  gptr<const float4> rows;
+  gptr<int> err;
+  gptr<uint32_t> cell;
+  gptr<float> fa, fb, ur, uphi, uz;
+  gptr<const float> w;
+};
+static_assert(sizeof(F32ArgsG) == sizeof(F32Args) && offsetof(F32ArgsG, er) == offsetof(F32Args, er) &&
+                  offsetof(F32ArgsG, w) == offsetof(F32Args, w) && offsetof(F32ArgsG, n) == offsetof(F32Args, n),
+              "F32ArgsG mirrors F32Args");
+struct GlobalSinkG {
+  int nr, nz;
+  gptr<double> jr, jphi, jz, rho;
+  gptr<int> err;
+};
+static_assert(sizeof(GlobalSinkG) == sizeof(GlobalSink) && offsetof(GlobalSinkG, err) == offsetof(GlobalSink, err),
+              "GlobalSinkG mirrors GlobalSink");
+
+__device__ __forceinline__ F32Args from_global(const F32ArgsG &g) {
+  F32Args a;
+  a.nr = g.nr;
+  a.nz = g.nz;
+  a.hr = g.hr;
+  a.hz = g.hz;
+  a.inv_c2 = g.inv_c2;
+  a.c1 = g.c1;
+  a.bscale = g.bscale;
+  a.escale = g.escale;
+  a.kz = g.kz;
+  a.kr = g.kr;
+  a.kp = g.kp;
+  a.pusher = g.pusher;
+  a.is_bunch = g.is_bunch;
+  a.want_rho = g.want_rho;
+  a.n = g.n;
+  a.er = (const double *)g.er;
+  a.ephi = (const double *)g.ephi;
+  a.ez = (const double *)g.ez;
+  a.hhr = (const double *)g.hhr;
+  a.hhphi = (const double *)g.hhphi;
+  a.hhz = (const double *)g.hhz;
+  a.rows = (const float4 *)g.rows;
+  a.err = (int *)g.err;
+  a.cell = (uint32_t *)g.cell;
+  a.fa = (float *)g.fa;
+  a.fb = (float *)g.fb;
+  a.ur = (float *)g.ur;
+  a.uphi = (float *)g.uphi;
+  a.uz = (float *)g.uz;
+  a.w = (const float *)g.w;
+  return a;
+}
+__device__ __forceinline__ GlobalSink from_global(const GlobalSinkG &g) {
+  GlobalSink s;
+  s.nr = g.nr;
+  s.nz = g.nz;
+  s.jr = (double *)g.jr;
+  s.jphi = (double *)g.jphi;
+  s.jz = (double *)g.jz;
+  s.rho = (double *)g.rho;
+  s.err = (int *)g.err;
+  return s;
+}
+
 template <int PPT, int WAVES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// 5 waves per SIMD (96 VGPRs, a few spilled) once the table pointers are
+// global; with flat pointers it needed 121 VGPRs and ran at 4
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_ens_fused32(const F32Args *__restrict__ args, const GlobalSink *__restrict__ sinks,
               const EnsItem *__restrict__ items, int n_items) {
   __shared__ double box[TileSink::kSize];
   __shared__ __align__(16) float fbox[kFBoxFloats];
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const EnsItem item = items[it];
-    fused32_tile<PPT, WAVES>(args[item.arg], sinks[item.arg], item.base, item.n_cap, box, fbox);
+    const F32ArgsG *ag = reinterpret_cast<const F32ArgsG *>(args) + item.arg;
+    const GlobalSinkG *sg = reinterpret_cast<const GlobalSinkG *>(sinks) + item.arg;
+    fused32_tile<PPT, WAVES>(from_global(*ag), from_global(*sg), item.base, item.n_cap, box, fbox);
     __syncthreads();  // the next item clears the boxes
   }
 }

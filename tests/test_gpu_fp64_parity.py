@@ -28,13 +28,17 @@ def _small_geometry():
     return dict(n_grid_r=33, n_grid_z=129, r_size=32 * 5e-4, z_size=128 * 5e-4)
 
 
-def test_field_updates_bitwise(capi, orc):
-    g = _small_geometry()
+# node counts: the small geometry, and grids whose sizes leave partial field
+# tiles in both directions with several tiles along z (tiles of 2 x 256 nodes
+# for H, 1 x 256 for E)
+@pytest.mark.parametrize("nr,nz", [(33, 129), (34, 600), (7, 1030)])
+def test_field_updates_bitwise(capi, orc, nr, nz):
+    g = dict(n_grid_r=nr, n_grid_z=nz, r_size=(nr - 1) * 5e-4, z_size=(nz - 1) * 5e-4)
     dt = 0.5 * 5e-4 / 3e8
     rng = np.random.default_rng(1)
     gpu = capi.Simulation(g["n_grid_r"], g["n_grid_z"], g["r_size"], g["z_size"], dt)
     ora = orc.Sim(g["n_grid_r"], g["n_grid_z"], g["r_size"], g["z_size"], dt)
-    pml = ((6.5) / 129, (6.5) / 129, (4.5) / 33, 1e-5, 7e-2)
+    pml = ((6.5) / nz, (6.5) / nz, (min(4, nr - 3) + 0.5) / nr, 1e-5, 7e-2)
     gpu.set_pml(*pml)
     ora.set_pml(*pml)
     assert np.array_equal(gpu.field("sigma"), ora.field("sigma"))
