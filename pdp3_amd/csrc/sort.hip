@@ -170,37 +170,58 @@ __device__ __forceinline__ uint64_t wave_runs(uint32_t key, bool valid, int &hea
   return heads;
 }
 
-// The run kernels take one chunk per block.  The chunk's keys are evaluated
-// in slot order (coalesced) and transposed to logical order in LDS (rows
-// padded to 17 words, so that neither direction has bank conflicts); the
-// block then walks them in groups of 256 logical keys, one record count
-// (block_runs / block_base entry) per group.
-constexpr int kGroups = kChunk / 256;
+// The run kernels take kCPB chunks per block (their loads all in flight
+// before the block's one barrier).  A chunk's keys are evaluated in slot order
+// (coalesced) and transposed to logical order in LDS (rows padded to 17
+// words, so that neither direction has bank conflicts); the block then walks
+// them in groups of 256 logical keys, one record count (block_runs /
+// block_base entry) per group.  nch: the species' number of chunks.
+constexpr int kCPB = 2;
+constexpr int kGroups = kChunk / 256;         // groups per chunk
+constexpr int kBlockGroups = kCPB * kGroups;  // groups per block
 constexpr int kRow = kPPT + 1;
 constexpr int kChunkLds = 64 * kRow;
 
+// the number of this block's chunks (the last block may hold fewer)
+__device__ __forceinline__ int block_chunks(long long nch) {
+  return (int)min((long long)kCPB, nch - (long long)blockIdx.x * kCPB);
+}
 template <class K>
-__device__ __forceinline__ void chunk_keys(const K &keys, long long c0, uint32_t *t) {
-  for (int m = 0; m < kGroups; m++) {
-    const int w = m * 256 + (int)threadIdx.x;  // slot offset: iteration w / 64 of lane w % 64
-    t[(w % 64) * kRow + w / 64] = keys(c0 + w);
-  }
+__device__ __forceinline__ void chunk_keys(const K &keys, int nc, uint32_t *t) {
+  const long long c0 = (long long)blockIdx.x * kCPB * kChunk;
+  uint32_t v[kCPB * kGroups];
+#pragma unroll
+  for (int c = 0; c < kCPB; c++)
+#pragma unroll
+    for (int m = 0; m < kGroups; m++)
+      v[c * kGroups + m] = c < nc ? keys(c0 + (long long)c * kChunk + m * 256 + (int)threadIdx.x) : 0u;
+#pragma unroll
+  for (int c = 0; c < kCPB; c++)
+#pragma unroll
+    for (int m = 0; m < kGroups; m++) {
+      const int w = m * 256 + (int)threadIdx.x;  // slot offset: iteration w / 64 of lane w % 64
+      t[c * kChunkLds + (w % 64) * kRow + w / 64] = v[c * kGroups + m];
+    }
   __syncthreads();
 }
-// logical offset s: lane s / kPPT, iteration s % kPPT
-__device__ __forceinline__ uint32_t chunk_key(const uint32_t *t, int s) { return t[(s / kPPT) * kRow + s % kPPT]; }
+// logical offset s in the block's chunks: chunk s / kChunk, and in it lane
+// (s % kChunk) / kPPT, iteration s % kPPT
+__device__ __forceinline__ uint32_t chunk_key(const uint32_t *t, int s) {
+  const int c = s / kChunk, l = s % kChunk;
+  return t[c * kChunkLds + (l / kPPT) * kRow + l % kPPT];
+}
 
-// the runs of the block's kGroups groups of 256 logical keys: per lane the
-// key and its run (head lane, length) in each group, and per group and wave
-// the number of runs in wc[group * 4 + wave] (LDS, read after a barrier)
+// the runs of the block's groups of 256 logical keys: per lane the key and
+// its run (head lane, length) in each group, and per group and wave the
+// number of runs in wc[group * 4 + wave] (LDS, read after a barrier)
 struct ChunkRuns {
-  uint32_t key[kGroups];
-  uint64_t heads[kGroups];
-  int head[kGroups], len[kGroups];
+  uint32_t key[kBlockGroups];
+  uint64_t heads[kBlockGroups];
+  int head[kBlockGroups], len[kBlockGroups];
 };
 __device__ __forceinline__ void chunk_runs(const uint32_t *t, ChunkRuns &cr, uint32_t *wc) {
 #pragma unroll
-  for (int m = 0; m < kGroups; m++) {
+  for (int m = 0; m < kBlockGroups; m++) {
     cr.key[m] = chunk_key(t, m * 256 + (int)threadIdx.x);
     cr.heads[m] = wave_runs(cr.key[m], true, cr.head[m], cr.len[m]);
     if (__lane_id() == 0) wc[m * 4 + (threadIdx.x >> 6)] = (uint32_t)__popcll(cr.heads[m]);
@@ -220,32 +241,35 @@ __device__ __forceinline__ uint32_t run_index(const ChunkRuns &cr, int m, uint32
 
 // runs per group of 256 logical keys
 template <class K>
-__global__ void __launch_bounds__(256) k_run_count(K keys, uint32_t *__restrict__ block_runs) {
-  __shared__ uint32_t t[kChunkLds];
-  __shared__ uint32_t wc[kGroups * 4];
-  chunk_keys(keys, (long long)blockIdx.x * kChunk, t);
+__global__ void __launch_bounds__(256) k_run_count(K keys, long long nch, uint32_t *__restrict__ block_runs) {
+  __shared__ uint32_t t[kCPB * kChunkLds];
+  __shared__ uint32_t wc[kBlockGroups * 4];
+  const int nc = block_chunks(nch);
+  chunk_keys(keys, nc, t);
   ChunkRuns cr;
   chunk_runs(t, cr, wc);
-  if (threadIdx.x < kGroups) {
+  if (threadIdx.x < nc * kGroups) {
     const uint32_t *g = wc + threadIdx.x * 4;
-    block_runs[(long long)blockIdx.x * kGroups + threadIdx.x] = g[0] + g[1] + g[2] + g[3];
+    block_runs[(long long)blockIdx.x * kBlockGroups + threadIdx.x] = g[0] + g[1] + g[2] + g[3];
   }
 }
 
 // one record (key, index, length) per run
 template <class K>
-__global__ void __launch_bounds__(256) k_run_emit(K keys, const uint32_t *__restrict__ block_base,
+__global__ void __launch_bounds__(256) k_run_emit(K keys, long long nch, const uint32_t *__restrict__ block_base,
                                                   const uint32_t *count, long long cap, uint32_t *__restrict__ rkey,
                                                   uint32_t *__restrict__ ridx, uint32_t *__restrict__ rlen) {
   if (*count > cap) return;  // more runs than record slots: this sort is skipped
-  __shared__ uint32_t t[kChunkLds];
-  __shared__ uint32_t wc[kGroups * 4];
-  chunk_keys(keys, (long long)blockIdx.x * kChunk, t);
+  __shared__ uint32_t t[kCPB * kChunkLds];
+  __shared__ uint32_t wc[kBlockGroups * 4];
+  const int nc = block_chunks(nch);
+  chunk_keys(keys, nc, t);
   ChunkRuns cr;
   chunk_runs(t, cr, wc);
 #pragma unroll
-  for (int m = 0; m < kGroups; m++) {
-    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kGroups + m], wc);
+  for (int m = 0; m < kBlockGroups; m++) {
+    if (m >= nc * kGroups) break;
+    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kBlockGroups + m], wc);
     if (cr.head[m] == (int)__lane_id()) {
       rkey[r] = cr.key[m];
       ridx[r] = r;
@@ -273,23 +297,25 @@ __global__ void k_run_first(const uint32_t *__restrict__ order, const uint32_t *
 // position (consecutive ranks are written by consecutive lanes); a skipped
 // sort leaves every particle in its own slot
 template <class K>
-__global__ void __launch_bounds__(256) k_run_place(K keys, const uint32_t *__restrict__ block_base,
+__global__ void __launch_bounds__(256) k_run_place(K keys, long long nch, const uint32_t *__restrict__ block_base,
                                                    const uint32_t *count, long long cap,
                                                    const uint32_t *__restrict__ rfirst, uint32_t *__restrict__ src) {
-  const long long c0 = (long long)blockIdx.x * kChunk;
+  const long long c0 = (long long)blockIdx.x * kCPB * kChunk;
+  const int nc = block_chunks(nch);
   if (*count > cap) {
-    for (int s = threadIdx.x; s < kChunk; s += 256) src[c0 + s] = (uint32_t)phys_of(c0 + s);
+    for (int s = threadIdx.x; s < nc * kChunk; s += 256) src[c0 + s] = (uint32_t)phys_of(c0 + s);
     return;
   }
-  __shared__ uint32_t t[kChunkLds];
-  __shared__ uint32_t wc[kGroups * 4];
-  chunk_keys(keys, c0, t);
+  __shared__ uint32_t t[kCPB * kChunkLds];
+  __shared__ uint32_t wc[kBlockGroups * 4];
+  chunk_keys(keys, nc, t);
   ChunkRuns cr;
   chunk_runs(t, cr, wc);
   const unsigned lane = __lane_id();
 #pragma unroll
-  for (int m = 0; m < kGroups; m++) {
-    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kGroups + m], wc);
+  for (int m = 0; m < kBlockGroups; m++) {
+    if (m >= nc * kGroups) break;
+    const uint32_t r = run_index(cr, m, block_base[(long long)blockIdx.x * kBlockGroups + m], wc);
     const int head = cr.head[m];
     const uint32_t first = head == (int)lane ? rfirst[r] : 0u;
     const uint32_t rank = __shfl(first, head) + (lane - (unsigned)head);
@@ -523,18 +549,19 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
   // runs per group of 256 logical keys and their scan (block[nb] = the
   // number of runs); n is a whole number of chunks, one per block
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, c.stream, block + nb, 1LL, 0u);
-  const unsigned chunks = (unsigned)(n / kChunk);
+  const long long nch = n / kChunk;
+  const unsigned chunks = (unsigned)nch, rblocks = (unsigned)((nch + kCPB - 1) / kCPB);
   if (f32)
-    hipLaunchKernelGGL(k_run_count<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block);
+    hipLaunchKernelGGL(k_run_count<KeysF32>, dim3(rblocks), dim3(256), 0, c.stream, k32, nch, block);
   else
-    hipLaunchKernelGGL(k_run_count<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block);
+    hipLaunchKernelGGL(k_run_count<KeysF64>, dim3(rblocks), dim3(256), 0, c.stream, k64, nch, block);
   exclusive_scan(block, nb + 1, tmp, c.stream);
   if (f32)
-    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block, count, cap, rk, ri,
-                       rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF32>, dim3(rblocks), dim3(256), 0, c.stream, k32, nch, block, count, cap, rk,
+                       ri, rlen);
   else
-    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block, count, cap, rk, ri,
-                       rlen);
+    hipLaunchKernelGGL(k_run_emit<KeysF64>, dim3(rblocks), dim3(256), 0, c.stream, k64, nch, block, count, cap, rk,
+                       ri, rlen);
   // records by key (stable), then each run's first rank
   radix_pairs(rk, ri, rk2, ri2, count, cap, bits, tmp, tmp + rec_hist, c.stream);
   const unsigned mb = (unsigned)((cap + 255) / 256);
@@ -543,9 +570,11 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
   exclusive_scan(lens, cap, tmp, c.stream);
   hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, c.stream, ri, lens, count, cap, rlen);
   if (f32)
-    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3(chunks), dim3(256), 0, c.stream, k32, block, count, cap, rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF32>, dim3(rblocks), dim3(256), 0, c.stream, k32, nch, block, count, cap, rlen,
+                       src);
   else
-    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3(chunks), dim3(256), 0, c.stream, k64, block, count, cap, rlen, src);
+    hipLaunchKernelGGL(k_run_place<KeysF64>, dim3(rblocks), dim3(256), 0, c.stream, k64, nch, block, count, cap, rlen,
+                       src);
   // the run count for the next sort's capacity
   if (hipMemcpyAsync(sp.runs_host, count, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
       hipEventRecord(sp.runs_ev, c.stream) != hipSuccess)
