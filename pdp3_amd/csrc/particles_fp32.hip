@@ -718,13 +718,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 // one block of a bunch (plain layout): WAVES chunks of kChunk slots from
 // `base`; the register / LDS sink is anchored at the smallest cell of the
 // block's living particles (a bunch block is a thin disc injected in one step)
-template <int WAVES>
+template <int WAVES, int PPT>
 __device__ __forceinline__ void fused32_bunch_tile(const F32Args &p, const GlobalSink &gs, long long base,
                                                    long long n_cap, double *box) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long cbase = base + (long long)wave * kChunk + lane;
+  const long long cbase = base + (long long)wave * (64 * PPT) + lane;
   int ai = 0x7fffffff, ak = 0x7fffffff;
-  for (int j = 0; j < kPPT; j++) {
+  for (int j = 0; j < PPT; j++) {
     long long n = cbase + (long long)j * 64;
     if (n < n_cap) {
       uint32_t c = p.cell[n];
@@ -740,9 +740,9 @@ __device__ __forceinline__ void fused32_bunch_tile(const F32Args &p, const Globa
   box_init(box, ai, ak, sk, gs);
   __syncthreads();
   const F32Args q = from_slot(p, base);
-  const unsigned lim = (unsigned)min(n_cap - base, (long long)WAVES * kChunk);
-  const unsigned off0 = (unsigned)(wave * kChunk + lane);
-  for (int j = 0; j < kPPT; j++) {
+  const unsigned lim = (unsigned)min(n_cap - base, (long long)WAVES * 64 * PPT);
+  const unsigned off0 = (unsigned)(wave * 64 * PPT + lane);
+  for (int j = 0; j < PPT; j++) {
     const unsigned n = off0 + (unsigned)j * 64u;
     if (n < lim) step_particle32(q, sk, GlobalFields(), n);
   }
@@ -751,9 +751,13 @@ __device__ __forceinline__ void fused32_bunch_tile(const F32Args &p, const Globa
   box_flush(box, sk);
 }
 
+// bunches: PPT particles per lane (wave w of a block takes the 64 * PPT slots
+// from w * 64 * PPT); a small bunch takes few per lane so that its blocks
+// still spread over the GPU
+template <int PPT>
 __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs, long long n_cap) {
   __shared__ double box[BunchSink::kSize];
-  fused32_bunch_tile<kWavesPerBlock>(p, gs, (long long)blockIdx.x * kBlockParticles, n_cap, box);
+  fused32_bunch_tile<kWavesPerBlock, PPT>(p, gs, (long long)blockIdx.x * (kWavesPerBlock * 64 * PPT), n_cap, box);
 }
 
 // ensembles: the plasma tiles and bunch blocks of every instance in one
@@ -1046,8 +1050,18 @@ void launch_fused_f32(Ctx &c, Species &sp) {
   if (sp.chunked) {
     launch_tile(c.tile, c.stream, args32(c, sp), sink32(c), sp.n_cap);
   } else {
-    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
-    hipLaunchKernelGGL(k_fused32_bunch, dim3(blocks), dim3(256), 0, c.stream, args32(c, sp), sink32(c), sp.n_cap);
+    // the most particles per lane that still give 1024 blocks
+    const long long n = sp.n_cap, per = (long long)kWavesPerBlock * 64;
+    const F32Args a = args32(c, sp);
+    const GlobalSink g = sink32(c);
+    if (n >= 1024 * per * kPPT)
+      hipLaunchKernelGGL(k_fused32_bunch<kPPT>, dim3((unsigned)((n + per * kPPT - 1) / (per * kPPT))), dim3(256), 0,
+                         c.stream, a, g, n);
+    else if (n >= 1024 * per * 4)
+      hipLaunchKernelGGL(k_fused32_bunch<4>, dim3((unsigned)((n + per * 4 - 1) / (per * 4))), dim3(256), 0, c.stream,
+                         a, g, n);
+    else
+      hipLaunchKernelGGL(k_fused32_bunch<1>, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, c.stream, a, g, n);
   }
 }
 
