@@ -691,15 +691,18 @@ k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s, long long n_ca
 }
 
 // bunches (plain layout): the tiled walk with the box anchored at the
-// smallest living cell of the block
+// smallest living cell of the block; PPT particles per lane (wave w of a
+// block takes the 64 * PPT slots from w * 64 * PPT), few for a small bunch so
+// that its blocks still spread over the GPU
+template <int PPT>
 __global__ void __launch_bounds__(256) k_fused_bunch(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s,
                                                      long long n_cap, int pusher) {
   __shared__ double box[BunchSink::kSize];
-  const long long base = (long long)blockIdx.x * kBlockParticles;
+  const long long base = (long long)blockIdx.x * (kWavesPerBlock * 64 * PPT);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long cbase = base + (long long)wave * kChunk + lane;
+  const long long cbase = base + (long long)wave * (64 * PPT) + lane;
   int ai = 0x7fffffff, ak = 0x7fffffff;
-  for (int j = 0; j < kPPT; j++) {
+  for (int j = 0; j < PPT; j++) {
     long long p = cbase + (long long)j * 64;
     if (p < n_cap && s.alive[p] == 1) {
       ai = min(ai, max(0, cell_number(s.r[p], g.dr)));
@@ -711,7 +714,7 @@ __global__ void __launch_bounds__(256) k_fused_bunch(GridDesc g, Fields f, Globa
   BunchSink sk;
   box_init(box, ai, ak, sk, gs);
   __syncthreads();
-  for (int j = 0; j < kPPT; j++) {
+  for (int j = 0; j < PPT; j++) {
     long long p = cbase + (long long)j * 64;
     if (p < n_cap) step_particle(g, f, sk, gs.err, s, pusher, true, p);
   }
@@ -852,9 +855,17 @@ void launch_fused_f64(Ctx &c, Species &sp) {
     hipLaunchKernelGGL(k_fused_tiled, dim3(blocks), dim3(256), 0, c.stream, c.g, fields_of(c), sink_of(c),
                        args_of(sp), sp.n_cap, c.pusher);
   } else {
-    unsigned blocks = (unsigned)((sp.n_cap + kBlockParticles - 1) / kBlockParticles);
-    hipLaunchKernelGGL(k_fused_bunch, dim3(blocks), dim3(256), 0, c.stream, c.g, fields_of(c), sink_of(c),
-                       args_of(sp), sp.n_cap, c.pusher);
+    // the most particles per lane that still give 1024 blocks
+    const long long n = sp.n_cap, per = (long long)kWavesPerBlock * 64;
+    if (n >= 1024 * per * kPPT)
+      hipLaunchKernelGGL(k_fused_bunch<kPPT>, dim3((unsigned)((n + per * kPPT - 1) / (per * kPPT))), dim3(256), 0,
+                         c.stream, c.g, fields_of(c), sink_of(c), args_of(sp), n, c.pusher);
+    else if (n >= 1024 * per * 4)
+      hipLaunchKernelGGL(k_fused_bunch<4>, dim3((unsigned)((n + per * 4 - 1) / (per * 4))), dim3(256), 0, c.stream,
+                         c.g, fields_of(c), sink_of(c), args_of(sp), n, c.pusher);
+    else
+      hipLaunchKernelGGL(k_fused_bunch<1>, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, c.stream, c.g,
+                         fields_of(c), sink_of(c), args_of(sp), n, c.pusher);
   }
 }
 
