@@ -112,6 +112,11 @@ class BumpRampProblem:
         self.bunch_r = np.maximum(self.bunch_r, max(s.dr / 4, s.bunch_r_min))
         self.bunch_z = rng.uniform(z_a, z_b, n)
         self.bunch_vr = rng.uniform(-1.0, 1.0, n) * s.bunch_vr_spread
+        # the same samples in order of z: the bunch kernel gathers and deposits
+        # a block's particles through an LDS box anchored at their smallest
+        # cell, which only helps when a block's particles lie close together
+        order = np.argsort(self.bunch_z, kind="stable")
+        self.bunch_r, self.bunch_z, self.bunch_vr = self.bunch_r[order], self.bunch_z[order], self.bunch_vr[order]
         # each macro-particle carries the volume pi R^2 (z_b - z_a) / n
         self.bunch_vol = PI * s.bunch_radius ** 2 * (z_b - z_a) / n
         centres = s.bump_start + np.arange(s.n_bumps) * (s.bump_span / max(s.n_bumps - 1, 1))
