@@ -193,7 +193,10 @@ int pdp3_sync(pdp3_ctx *ctx);
  * sort_interval steps; 0 disables.  The periodic sorts do not wait for the
  * device: their record space follows the previous sort's count of runs of
  * equal cells, and a sort that finds more runs than that leaves the order as
- * it is (the next one is sized for it).  pdp3_sort_particles always sorts. */
+ * it is (the next one is sized for it).  Their buffers (a second copy of the
+ * species' particle arrays and a workspace) are allocated by the first step
+ * taken with sort_interval > 0, so later steps allocate nothing.
+ * pdp3_sort_particles always sorts. */
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
 
@@ -256,7 +259,8 @@ const char *pdp3_ensemble_last_error(const pdp3_ensemble *e);
 int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps);
 /* cell sort of every instance every sort_interval steps (outside the graph;
  * the periodic sorts of pdp3_set_sort_interval, enqueued for all instances
- * before any is waited for) */
+ * before any is waited for; their buffers are allocated by the first
+ * pdp3_ensemble_step call after it is set) */
 int pdp3_ensemble_set_sort_interval(pdp3_ensemble *e, int sort_interval);
 /* device time of the stepped graphs since the last call (HIP events), and the step count */
 int pdp3_ensemble_timing(pdp3_ensemble *e, double *total_ms, long long *steps);
