@@ -195,7 +195,8 @@ int pdp3_sync(pdp3_ctx *ctx);
  * equal cells, and a sort that finds more runs than that leaves the order as
  * it is (the next one is sized for it).  Their buffers (a second copy of the
  * species' particle arrays and a workspace) are allocated by the first step
- * taken with sort_interval > 0, so later steps allocate nothing.
+ * taken with sort_interval > 0; later steps allocate only when the record
+ * space has to grow.
  * pdp3_sort_particles always sorts. */
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
