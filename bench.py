@@ -145,6 +145,19 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
 
 
+def describe_workload(w):
+    """The workload named by its sizes: grid, species, bunch, PML, pusher."""
+    parts = []
+    for p in w.plasma:
+        kind = "electrons" if p.charge < 0 else f"ions (charge {p.charge}, mass {p.mass:g} m_e)"
+        dens = (f"0->{p.density:g} m^-3 ramp over {p.ramp_cells} z-cells" if p.ramp_cells
+                else f"{p.density:g} m^-3")
+        parts.append(f"{p.ppc} ppc {kind} ({w.plasma_count(p)}), {dens}")
+    bunch = ", ".join(f"bunch of {len(b.r)} macro-particles" for b in w.bunches)
+    return (f"{w.name}: r-z {w.cells_r}x{w.cells_z} cells, dr=dz={w.dr} m, " + "; ".join(parts)
+            + (f"; {bunch}" if bunch else "") + f"; PML {w.pml_cells}, {'Vay' if w.pusher == 1 else 'Boris'}")
+
+
 def traffic_from_profile(name):
     """HBM bytes per launch of the dominant kernel as measured by rocprofv3
     (FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a
@@ -330,15 +343,19 @@ def main():
         err["particles"] = momentum_error(wl, capi, a.config, min(1024, w.cells_z), a.steps + a.warmup)
         print(f"# particle errors: {err['particles']}", file=sys.stderr, flush=True)
         p64 = r64["particle_ms"] / max(r64["particle_launches"], 1)
+        n_sp = max(len(w.plasma), 1)
         fp64 = dict(value=v64, ms_per_step=1e3 * r64["wall_s"] / a.steps, particle_kernel_ms=p64,
-                    roofline_frac=(n_plasma * BYTES_PER_PARTICLE["fp64"] + cells * BYTES_PER_CELL) / (p64 * 1e-3)
-                    / 1e9 / HBM_PEAK_GBS)
+                    roofline_frac=(n_plasma * BYTES_PER_PARTICLE["fp64"] + n_sp * cells * BYTES_PER_CELL) / n_sp
+                    / (p64 * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
+    # one launch per plasma species and step: the average launch moves its
+    # species' particles and the grid once
+    n_species = max(len(w.plasma), 1)
     per_launch_ms = r32["particle_ms"] / max(r32["particle_launches"], 1)
-    bytes_launch = n_plasma * BYTES_PER_PARTICLE["fp32"] + cells * BYTES_PER_CELL
+    bytes_launch = (n_plasma * BYTES_PER_PARTICLE["fp32"] + n_species * cells * BYTES_PER_CELL) / n_species
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=traffic_from_profile("k_fused32_tiled_cfg3"), kernel="k_fused32_tiled (plasma species)",
+                    traffic=traffic_from_profile(f"k_fused32_tiled_{a.config}"), kernel="k_fused32_tiled (plasma species)",
                     bytes_per_launch=bytes_launch, avg_launch_ms=per_launch_ms)
 
     cpu = None
@@ -351,9 +368,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"{w.name}: r-z {w.cells_r}x{w.cells_z} cells, dr=dz={w.dr} m, "
-                               f"{w.plasma[0].ppc} ppc electrons ({n_plasma}) with a 0->1e18 m^-3 ramp over "
-                               f"1000 z-cells, Gaussian 0.999c bunch, PML {w.pml_cells}, Boris",
+        "config": {"workload": describe_workload(w),
                    "precision_form": "fp32 cell-relative (FP64 grid sums, axis cell, fields)",
                    "sort_interval": a.sort_interval},
         "fp64_plain": fp64,
