@@ -39,10 +39,22 @@ BYTES_PER_PARTICLE = {"fp32": 52.0, "fp64": 90.0}
 BYTES_PER_CELL = 72.0
 
 
+AUTOTUNE_STEPS = 12  # 6 tile shapes x 2 steps
+
+
 def run_form(wl, capi, w, precision, steps, warmup, sort_interval):
     sim = wl.build_simulation(w, precision=precision, sort_interval=sort_interval)
     sim.sync()
     sim.step(warmup)
+    sim.sync()
+    # the library's autotuner picks the tile shape of the FP32 particle kernel
+    # (AUTOTUNE_STEPS real steps: 2 per shape); the FP64 form has one shape and
+    # takes the same steps, so that both forms are compared at one time
+    if precision == capi.PREC_FP32:
+        tile = list(sim.autotune(AUTOTUNE_STEPS // 6))
+    else:
+        tile = None
+        sim.step(AUTOTUNE_STEPS)
     sim.sync()
     sim.profile(True)
     sim.profile_reset()
@@ -58,7 +70,7 @@ def run_form(wl, capi, w, precision, steps, warmup, sort_interval):
     sim.profile(False)
     ez = sim.field("E_z")
     out = dict(wall_s=t1 - t0, particle_ms=ms, particle_launches=launches, bunch_ms=ms_b, h_ms=ms_h, e_ms=ms_e,
-               sort_ms=ms_s, sort_launches=ls)
+               sort_ms=ms_s, sort_launches=ls, tile=tile)
     sim.close()
     return out, ez
 
@@ -339,7 +351,8 @@ def main():
         den = np.sqrt(np.mean(ez64 ** 2))
         err = dict(E_z_rel_rms=float(np.sqrt(np.mean((ez32 - ez64) ** 2)) / den),
                    E_z_rel_max=float(np.max(np.abs(ez32 - ez64)) / np.max(np.abs(ez64))),
-                   after_steps=a.steps + a.warmup, reference="FP64 plain form on the GPU (oracle-parity checked)")
+                   after_steps=a.steps + a.warmup + AUTOTUNE_STEPS,
+                   reference="FP64 plain form on the GPU (oracle-parity checked)")
         err["particles"] = momentum_error(wl, capi, a.config, min(1024, w.cells_z), a.steps + a.warmup)
         print(f"# particle errors: {err['particles']}", file=sys.stderr, flush=True)
         p64 = r64["particle_ms"] / max(r64["particle_launches"], 1)
@@ -370,7 +383,7 @@ def main():
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": describe_workload(w),
                    "precision_form": "fp32 cell-relative (FP64 grid sums, axis cell, fields)",
-                   "sort_interval": a.sort_interval},
+                   "sort_interval": a.sort_interval, "tile_ppt_waves": r32["tile"]},
         "fp64_plain": fp64,
         "speedup_fp32_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "error_fp32_vs_fp64": err,
