@@ -75,6 +75,22 @@ def run_form(wl, capi, w, precision, steps, warmup, sort_interval):
     return out, ez
 
 
+def ez_error_full(wl, capi, w, steps, sort_interval):
+    """E_z of the FP32 form against the FP64 plain form on the full grid after
+    `steps` steps from the same initial state, both forms with the same
+    periodic cell sort (the production loop)."""
+    ez = []
+    for p in (capi.PREC_FP32, capi.PREC_FP64):
+        sim = wl.build_simulation(w, precision=p, sort_interval=sort_interval)
+        sim.step(steps)
+        ez.append(sim.field("E_z"))
+        sim.close()
+    d = ez[0] - ez[1]
+    return dict(E_z_rel_rms=float(np.sqrt(np.mean(d ** 2) / np.mean(ez[1] ** 2))),
+                E_z_rel_max=float(np.max(np.abs(d)) / np.max(np.abs(ez[1]))), after_steps=steps,
+                sample=f"full {w.cells_r}x{w.cells_z} grid, cell sort every {sort_interval} steps")
+
+
 def momentum_error(wl, capi, name, cells_z, steps):
     """Particle momentum (gamma v) and E_z errors of the FP32 form against the
     FP64 plain form on a z-slab of the workload: the same spacing, ppc, density
@@ -108,14 +124,39 @@ def momentum_error(wl, capi, name, cells_z, steps):
                        f"{steps} steps, no cell sort")
 
 
-def cpu_baseline(wl, capi, w, threads):
+def host_cpu():
+    """(threads for the CPU baseline, logical CPUs visible, CPU model): the
+    process's CPU share, i.e. its affinity set, capped by OMP_NUM_THREADS when
+    that is set (one GPU's share of a multi-GPU host)"""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    threads = visible
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, visible, model
+
+
+def cpu_baseline(wl, capi, w, threads, visible, model, cells_z=1024, nsteps=3):
     """The oracle (FP64, reference-faithful: push/move loops threaded, deposits
     serial as in the reference) on a bounded sample of the workload: the same
-    cells in r, 128 cells in z, the same ppc and density, 3 steps."""
+    cells in r, the first cells_z cells in z at the same ppc and spacing
+    (uniform density), nsteps full steps (H, push, deposit, E)."""
     from oracle import oracle as orc
 
     s = wl.config(w.name, seed=w.seed)
-    s.cells_z = 128
+    s.cells_z = min(cells_z, w.cells_z)
     s.bunches = []
     for p in s.plasma:
         p.ramp_cells = 0
@@ -128,12 +169,13 @@ def cpu_baseline(wl, capi, w, threads):
         ora.set_pml(*pml)
     for p, st in zip(s.plasma, states):
         ora.add_plasma(p.charge, p.mass, st["r"], st["z"], st["vr"], st["vphi"], st["vz"], st["w"], st["alive"])
-    n = sum(len(st["r"]) for st in states)
-    nsteps = 3
+    del states
+    n = sum(s.plasma_count(p) for p in s.plasma)
     t0 = time.perf_counter()
     ora.step(nsteps)
     dt = time.perf_counter() - t0
     return dict(value=n * nsteps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                host={"cpu_model": model, "logical_cpus_visible": visible, "omp_threads": threads},
                 sample=f"{s.cells_r}x{s.cells_z}-cell slab of the same grid spacing, {n} particles "
                        f"({s.plasma[0].ppc} ppc), {nsteps} full steps, {dt:.1f} s")
 
@@ -299,6 +341,9 @@ def main():
                     help="cell sort of the plasma every N steps (counted in the timed steps); "
                          "default 20 for cfg3, 40 for the cfg4 ensemble")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-form comparison run")
+    ap.add_argument("--error-steps", type=int, default=None,
+                    help="steps of the FP32-vs-FP64 error leg (default: the configuration's own count, "
+                         "1000 for cfg3, 2000 for cfg2; 0 skips it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
@@ -353,7 +398,13 @@ def main():
                    E_z_rel_max=float(np.max(np.abs(ez32 - ez64)) / np.max(np.abs(ez64))),
                    after_steps=a.steps + a.warmup + AUTOTUNE_STEPS,
                    reference="FP64 plain form on the GPU (oracle-parity checked)")
-        err["particles"] = momentum_error(wl, capi, a.config, min(1024, w.cells_z), a.steps + a.warmup)
+        err_steps = w.steps if a.error_steps is None else a.error_steps
+        if err_steps > 0:
+            # at the configuration's own step count: E_z on the full grid, and
+            # momenta and E_z on a z-slab (no sort, particles pair by index)
+            err["at_config_steps"] = ez_error_full(wl, capi, w, err_steps, a.sort_interval)
+            print(f"# E_z error after {err_steps} steps: {err['at_config_steps']}", file=sys.stderr, flush=True)
+        err["particles"] = momentum_error(wl, capi, a.config, min(1024, w.cells_z), max(err_steps, a.steps + a.warmup))
         print(f"# particle errors: {err['particles']}", file=sys.stderr, flush=True)
         p64 = r64["particle_ms"] / max(r64["particle_launches"], 1)
         n_sp = max(len(w.plasma), 1)
@@ -373,8 +424,8 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, capi, w, threads)
+        threads, visible, model = host_cpu()
+        cpu = cpu_baseline(wl, capi, w, threads, visible, model)
         print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
 
     line = {

@@ -213,6 +213,21 @@ int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
  * doubles; R is 0 at k = 0, k = n_grid_z-1 and i = n_grid_r-1. */
 int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out);
 
+/* TemperatureClassic::calc_t + normalize of one species
+ * (src/temperatureClassic.cpp:57-158, the "T" probe of LoadInitParam::dump_data,
+ * src/loadInitParam.cpp:369-379), computed on the device from the resident
+ * particle state: per cell (CELL_NUMBER of r and z) the sums of |v|, v_r,
+ * v_phi, v_z and the count, then T = m v^2/2 / |charge| in eV with
+ * v^2 = (S^2 - |V|^2) / N^2 (m c^2 (gamma-1) / |charge| above 5e7 m/s).
+ * out: n_grid_r x n_grid_z; row n_grid_r-1 and column n_grid_z-1 are 0, an
+ * empty cell is NaN (0/0), as in the reference. */
+int pdp3_temperature(pdp3_ctx *ctx, int index, double *out);
+/* ParticlesDensity::calc_density of one species (src/particlesDensity.cpp:40-225,
+ * the "density" probe, src/loadInitParam.cpp:381-390) on the device: every
+ * particle's mass_array / EL_MASS spread over its four nodes with the
+ * reference's ring-volume weights.  out: n_grid_r x n_grid_z. */
+int pdp3_density(pdp3_ctx *ctx, int index, double *out);
+
 /* tile shape of the FP32 particle kernel (no reference counterpart):
  * particles walked per lane (ppt, 4 / 8 / 16) and wavefronts per block
  * (waves); supported pairs: (16,4) (16,2) (16,1) (8,4) (8,2) (4,4).  The
@@ -241,6 +256,32 @@ int pdp3_autotune(pdp3_ctx *ctx, int steps_per_candidate, int *ppt, int *waves);
  * sort_interval 0 (a fixed particle order). */
 int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo,
                      double z_hi, double *phi_out, double *const *dphi_dw);
+
+/* Tangent-linear model of the same loop and objective (no reference
+ * counterpart): runs nsteps iterations from the current state, like
+ * pdp3_objective_run, and returns phi and its directional derivatives along
+ * PDP3_TANGENT_DIRS directions of the particle weights.  dw[j] holds species
+ * j's PDP3_TANGENT_DIRS x n weight tangents, direction-major, in its logical
+ * particle order (a NULL entry: zero).  The tangents are carried forward
+ * through the FP64 plain form's particle step in dual numbers and through the
+ * field updates themselves, i.e. the linearisation that pdp3_adjoint_run
+ * transposes (around the context's own trajectory, FP32 or FP64 form): for
+ * every direction d, sum over particles of dphi_dw * dw[d] equals dphi_out[d]
+ * to round-off (the adjoint's dot-product test).  Requires sort_interval 0. */
+#define PDP3_TANGENT_DIRS 4
+int pdp3_tangent_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi,
+                     const double *const *dw, double *phi_out, double *dphi_out);
+/* pdp3_adjoint_run with pdp3_tangent_run's tangents carried along the
+ * adjoint's own forward sweep (no reference counterpart): the tangents follow
+ * exactly the trajectory the checkpoints record, which a separate run cannot
+ * (the grid sums use FP64 atomics, so two runs differ by round-off that the
+ * loop amplifies).  With checkpoint_every 1 the adjoint linearises around the
+ * same states, and the dot-product identity holds to the round-off of the
+ * sums; with longer segments the recomputed steps between checkpoints
+ * deviate by amplified round-off. */
+int pdp3_adjoint_tangent_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo,
+                             double z_hi, const double *const *dw, double *phi_out, double *const *dphi_dw,
+                             double *dphi_tangent);
 
 /* the same objective without the gradient (forward only) */
 int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi, double *phi_out);

@@ -54,6 +54,7 @@ def lib():
         L.or_sim_set_pusher.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_set_reverse_deposit.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_gamma_inv_nearest.argtypes = [C.c_int]
         L.or_sim_time.restype = C.c_double
         L.or_sim_time.argtypes = [C.c_void_p]
         L.or_sim_dr.restype = C.c_double
@@ -76,6 +77,8 @@ def lib():
         L.or_sim_step_v.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_full_current_distribution.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_charge_weighting.argtypes = [C.c_void_p, C.c_int, _D]
+        L.or_temperature.argtypes = [C.c_void_p, C.c_int, _D]
+        L.or_density.argtypes = [C.c_void_p, C.c_int, _D]
         L.or_sim_bunch_inject.argtypes = [C.c_void_p, C.c_int]
         L.or_sim_get_e.argtypes = [C.c_void_p, C.c_double, C.c_double, _D]
         L.or_sim_get_h.argtypes = [C.c_void_p, C.c_double, C.c_double, _D]
@@ -177,6 +180,22 @@ class Sim:
         lib().or_sim_get_species(self.h, idx, *(_dp(out[k]) for k in ("r", "z", "vr", "vphi", "vz", "q", "m")),
                                  alive.ctypes.data_as(_U8))
         out["alive"] = alive
+        return out
+
+    def temperature(self, idx):
+        """TemperatureClassic of species idx (nr x nz, eV)"""
+        out = np.zeros((self.nr, self.nz))
+        st = lib().or_temperature(self.h, idx, _dp(out))
+        if st != 0:
+            raise RuntimeError(f"oracle temperature failed with status {st}")
+        return out
+
+    def density(self, idx):
+        """ParticlesDensity::calc_density of species idx (nr x nz)"""
+        out = np.zeros((self.nr, self.nz))
+        st = lib().or_density(self.h, idx, _dp(out))
+        if st != 0:
+            raise RuntimeError(f"oracle density failed with status {st}")
         return out
 
     def step(self, n=1):

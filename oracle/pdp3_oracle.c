@@ -83,8 +83,33 @@ static int get_gamma(double sq_velocity, double *gamma) {
   *gamma = g;
   return OR_OK;
 }
+/* Test hook (off by default): or_set_gamma_inv_nearest(1) replaces glibc's
+ * pow(y, -0.5) by the double nearest to 1/sqrt(y), picked among 1/sqrt(y)
+ * and its two neighbours on the residual 1 - y c^2 evaluated with
+ * error-free products.  glibc's pow returns that double in all but a few
+ * hundredths of a percent of arguments; the switch measures how far the
+ * loop moves when only those last-bit cases change. */
+static int g_gamma_inv_nearest = 0;
+void or_set_gamma_inv_nearest(int on) { g_gamma_inv_nearest = on != 0; }
+static double rsqrt_residual(double y, double c) {
+  double p = c * c, pe = fma(c, c, -p);
+  double q = y * p, qe = fma(y, p, -q);
+  return ((1.0 - q) - qe) - y * pe;
+}
+static double rsqrt_nearest(double y) {
+  double c = 1.0 / sqrt(y), best = c, rb = fabs(rsqrt_residual(y, c));
+  double lo = nextafter(c, 0.0), hi = nextafter(c, 1e308);
+  double rl = fabs(rsqrt_residual(y, lo)), rh = fabs(rsqrt_residual(y, hi));
+  if (rl < rb) {
+    best = lo;
+    rb = rl;
+  }
+  if (rh < rb) best = hi;
+  return best;
+}
 static double get_gamma_inv(double sq_velocity) {
-  return pow(1.0 + sq_velocity / C2, -0.5);
+  double y = 1.0 + sq_velocity / C2;
+  return g_gamma_inv_nearest ? rsqrt_nearest(y) : pow(y, -0.5);
 }
 /* tinyvec3d::tv_dot (lib/tinyvec3d/tinyvec3d.cpp:54-61): ((0+a0b0)+a1b1)+a2b2 */
 static double dot3(const double *a, const double *b) {
@@ -1124,6 +1149,141 @@ int or_sim_charge_weighting(or_sim *s, int idx, double *rho) {
     if (!p->alive[i]) continue;
     if (!finite_pos(p->r[i], p->z[i])) return OR_ERR_NONFINITE;
     or_charge_deposit(s, p->q[i], p->r[i], p->z[i], rho);
+  }
+  return OR_OK;
+}
+
+/* ---- probes ---- */
+
+/* TemperatureClassic::calc_t + normalize, src/temperatureClassic.cpp:57-158.
+ * Per cell (CELL_NUMBER of r and z, :89-90) the sums of |v| (lib::sq_rt, the
+ * plain sqrt of the non-AVX build, src/lib.cpp:25-32), v_r, v_phi, v_z and
+ * the count over the living particles in index order; then per cell
+ * v^2 = (S^2 - |V|^2) / N^2 and T = m v^2 / 2 below REL_LIMIT^2 (5e7 m/s,
+ * :4-6), else m c^2 (gamma - 1) with lib::get_gamma (src/lib.cpp:45-70), in
+ * eV (divided by |charge|, :147).  m and charge are the species' per real
+ * particle.  out: nr x nz, row nr-1 and column nz-1 stay 0 (reset() clears
+ * them and normalize never writes them); an empty cell gives NaN (0/0). */
+int or_temperature(const or_sim *s, int idx, double *out) {
+  const species_t *p = &s->sp[idx];
+  const size_t cells = (size_t)s->nr * (size_t)s->nz;
+  double *sum = calloc(4 * cells, sizeof(double));
+  double *cnt = calloc(cells, sizeof(double));
+  if (!sum || !cnt) {
+    free(sum);
+    free(cnt);
+    return OR_ERR_OOM;
+  }
+  double *vr = sum + cells, *vphi = sum + 2 * cells, *vz = sum + 3 * cells;
+  for (int j = 0; j < p->n; j++) {
+    if (!p->alive[j]) continue;
+    if (!finite_pos(p->r[j], p->z[j])) {
+      free(sum);
+      free(cnt);
+      return OR_ERR_NONFINITE;
+    }
+    int i = or_cell_number(p->r[j], s->dr), k = or_cell_number(p->z[j], s->dz);
+    if (i < 0 || k < 0 || i >= s->nr || k >= s->nz) continue;
+    size_t c = (size_t)i * (size_t)s->nz + (size_t)k;
+    double a = p->vr[j], b = p->vphi[j], e = p->vz[j];
+    sum[c] += sqrt(a * a + b * b + e * e);
+    vr[c] += a;
+    vphi[c] += b;
+    vz[c] += e;
+    cnt[c] += 1;
+  }
+  const double rel_limit = 5e7;
+  for (size_t c = 0; c < cells; c++) out[c] = 0.0;
+  for (int i = 0; i < s->nr - 1; i++)
+    for (int k = 0; k < s->nz - 1; k++) {
+      size_t c = (size_t)i * (size_t)s->nz + (size_t)k;
+      double vec2 = vr[c] * vr[c] + vphi[c] * vphi[c] + vz[c] * vz[c];
+      double v2 = (sum[c] * sum[c] - vec2) / (cnt[c] * cnt[c]);
+      double t;
+      if (v2 < rel_limit * rel_limit) {
+        t = p->mass * v2 / 2;
+      } else {
+        double g;
+        if (get_gamma(v2, &g) != OR_OK) {
+          free(sum);
+          free(cnt);
+          return OR_ERR_SUPERLUMINAL;
+        }
+        t = p->mass * C2 * (g - 1);
+      }
+      out[c] = t / fabs(p->charge);
+    }
+  free(sum);
+  free(cnt);
+  return OR_OK;
+}
+
+/* ParticlesDensity::calc_density, src/particlesDensity.cpp:40-225: each living
+ * particle (index order) spreads ro_p = mass_array / EL_MASS / v_0 over the
+ * four nodes (i..i+1, k..k+1) with ring-volume weights; three radial branches
+ * (r > dr, r <= dr/2, else), each with its own v_0 and axis-cell volume, in
+ * the reference's expression order (CYL_RNG_VOL, CELL_VOLUME and CYL_VOL of
+ * include/geometry.h:33-39).  out: nr x nz, accumulated (the caller clears
+ * it, as ParticlesDensity::reset). */
+int or_density(const or_sim *s, int idx, double *out) {
+  const species_t *p = &s->sp[idx];
+  const double dr = s->dr, dz = s->dz;
+  const size_t nz = (size_t)s->nz;
+  for (int j = 0; j < p->n; j++) {
+    if (!p->alive[j]) continue;
+    double r = p->r[j], z = p->z[j];
+    if (!finite_pos(r, z)) return OR_ERR_NONFINITE;
+    int r_i = or_cell_number(r, dr), z_k = or_cell_number(z, dz);
+    if (r_i < 0 || z_k < 0 || r_i > s->nr - 2 || z_k > s->nz - 2) continue;
+    double r1, r2, r3, dz1, dz2, v_0, v_1, v_2, ro_p, w00, w10, w01, w11;
+    if (r > dr) {
+      r1 = r - 0.5 * dr;
+      r2 = (r_i + 0.5) * dr;
+      r3 = r + 0.5 * dr;
+      v_0 = 2. * PI * dz * dr * r;
+      ro_p = p->m[j] / EL_MASS / v_0;
+      v_1 = cell_volume(r_i, dr, dz);
+      v_2 = cell_volume(r_i + 1, dr, dz);
+      dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
+      dz2 = (z + 0.5 * dz) - (z_k + 0.5) * dz;
+      w00 = PI * (dz1) * ((r2) * (r2) - (r1) * (r1)) / v_1;
+      w10 = PI * (dz1) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+      w01 = PI * (dz2) * ((r2) * (r2) - (r1) * (r1)) / v_1;
+      w11 = PI * (dz2) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+    } else if (r <= dr / 2.) {
+      r_i = 0;
+      r2 = (r_i + 0.5) * dr;
+      r3 = r + 0.5 * dr;
+      dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
+      dz2 = (z + 0.5 * dz) - (z_k + 0.5) * dz;
+      v_0 = PI * dz * (2. * r * r + dr * dr / 2.);
+      ro_p = p->m[j] / EL_MASS / v_0;
+      v_1 = cyl_vol(dz, dr);
+      v_2 = cell_volume(r_i + 1, dr, dz);
+      w00 = PI * dz1 * (dr * dr / 2. - r * dr + r * r) / v_1;
+      w10 = PI * (dz1) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+      w01 = PI * dz2 * (dr * dr / 2. - r * dr + r * r) / v_1;
+      w11 = PI * (dz2) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+    } else {
+      r1 = r - 0.5 * dr;
+      r2 = (r_i + 0.5) * dr;
+      r3 = r + 0.5 * dr;
+      dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
+      dz2 = (z + 0.5 * dz) - (z_k + 0.5) * dz;
+      v_0 = 2. * PI * dz * dr * r;
+      ro_p = p->m[j] / EL_MASS / v_0;
+      v_1 = cyl_vol(dz, dr);
+      v_2 = cell_volume(r_i + 1, dr, dz);
+      w00 = PI * (dz1) * ((r2) * (r2) - (r1) * (r1)) / v_1;
+      w10 = PI * (dz1) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+      w01 = PI * (dz2) * ((r2) * (r2) - (r1) * (r1)) / v_1;
+      w11 = PI * (dz2) * ((r3) * (r3) - (r2) * (r2)) / v_2;
+    }
+    size_t c = (size_t)r_i * nz + (size_t)z_k;
+    out[c] += ro_p * w00;
+    out[c + nz] += ro_p * w10;
+    out[c + 1] += ro_p * w01;
+    out[c + nz + 1] += ro_p * w11;
   }
   return OR_OK;
 }

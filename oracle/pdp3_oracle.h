@@ -24,6 +24,7 @@ extern "C" {
 #define OR_ERR_NONFINITE (-4)
 #define OR_ERR_SUPERLUMINAL (-5)
 #define OR_ERR_BAD_ARG (-1)
+#define OR_ERR_OOM (-3)
 
 /* pushers (configure.ac:142-154; "boris" = PUSHER_BORIS_RELATIVISTIC, which is
  * also what the default adaptive build compiles to) */
@@ -61,6 +62,10 @@ void or_sim_set_threads(or_sim *s, int threads);
 /* test hook: run the serial deposit loops in reverse particle order, to
  * measure how much the loop's result depends on the order of the grid sums */
 void or_sim_set_reverse_deposit(or_sim *s, int reverse);
+/* test hook, process-wide: 1 takes lib::get_gamma_inv as the double nearest
+ * to 1/sqrt(y) instead of glibc's pow(y, -0.5) (they differ in the last bit
+ * for a few hundredths of a percent of arguments) */
+void or_set_gamma_inv_nearest(int on);
 
 /* plasma species (specular walls).  charge_e / mass_e in units of the
  * electron charge / mass as in parameters.xml (src/particles.cpp:30-33).
@@ -103,6 +108,12 @@ void or_sim_reset_j(or_sim *s);                      /* Current::reset_j */
 int or_sim_step_v(or_sim *s, int idx);               /* Particles::step_v */
 int or_sim_full_current_distribution(or_sim *s, int idx);
 int or_sim_charge_weighting(or_sim *s, int idx, double *rho);
+/* TemperatureClassic::calc_t + normalize (src/temperatureClassic.cpp:57-158)
+ * of species idx: nr x nz temperatures in eV (row nr-1, column nz-1: 0) */
+int or_temperature(const or_sim *s, int idx, double *out);
+/* ParticlesDensity::calc_density (src/particlesDensity.cpp:40-225) of
+ * species idx, added into out (nr x nz) */
+int or_density(const or_sim *s, int idx, double *out);
 void or_sim_bunch_inject(or_sim *s, int idx);
 
 /* point evaluations */

@@ -175,6 +175,29 @@ class BumpRampProblem:
             sim.close()
 
 
+    def directional_derivatives(self, theta, dthetas, precision=capi.PREC_FP64):
+        """phi(theta) and its derivatives along up to TANGENT_DIRS parameter
+        directions (rows of dthetas) in one forward-mode run of the library's
+        tangent-linear model (pdp3_tangent_run)"""
+        s = self.spec
+        dthetas = torch.as_tensor(dthetas, dtype=torch.float64).reshape(-1, len(theta))
+        k = dthetas.shape[0]
+        if k > capi.TANGENT_DIRS:
+            raise ValueError(f"at most {capi.TANGENT_DIRS} directions per run")
+        th = torch.as_tensor(theta, dtype=torch.float64).detach()
+        dws = [np.zeros((capi.TANGENT_DIRS, n)) for n in (len(self.plasma["r"]), len(self.bunch_r))]
+        for d in range(k):
+            _, tangents = torch.func.jvp(lambda t: tuple(self.weights(t)), (th,), (dthetas[d],))
+            for j, t in enumerate(tangents):
+                dws[j][d] = t.numpy()
+        sim = self.build(th, precision)
+        try:
+            phi, dphi = sim.tangent_run(s.steps, s.window_steps, *s.window, dws)
+        finally:
+            sim.close()
+        return phi, dphi[:k]
+
+
 class PicObjective(torch.autograd.Function):
     """phi = PicObjective.apply(theta, problem[, precision]): the objective of
     the r-z PIC loop; backward through the library's adjoint."""

@@ -54,6 +54,7 @@ UNITS = [
     ("particles_fp32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("sort.hip", []),
     ("adjoint.hip", ["-ffp-contract=off"]),
+    ("probes.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -24,6 +24,7 @@ COMPONENTS = {
     "rho_bunch": 12, "sigma": 13, "epsilon": 14,
 }
 KERNEL_PARTICLES, KERNEL_H, KERNEL_E, KERNEL_SORT, KERNEL_BUNCH = range(5)
+TANGENT_DIRS = 4  # PDP3_TANGENT_DIRS
 
 _D = C.POINTER(C.c_double)
 _U8 = C.POINTER(C.c_uint8)
@@ -67,9 +68,13 @@ SIGNATURES = {
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
+    "pdp3_temperature": (_I, [_P, _I, _D]),
+    "pdp3_density": (_I, [_P, _I, _D]),
     "pdp3_bunch_add_loaded": (_I, [_P, _I, _F, _I, _D, _D, _D, _D, _D, _D, C.POINTER(_I)]),
     "pdp3_adjoint_run": (_I, [_P, _I, _I, _I, _F, _F, _D, C.POINTER(_D)]),
     "pdp3_objective_run": (_I, [_P, _I, _I, _F, _F, _D]),
+    "pdp3_tangent_run": (_I, [_P, _I, _I, _F, _F, C.POINTER(_D), _D, _D]),
+    "pdp3_adjoint_tangent_run": (_I, [_P, _I, _I, _I, _F, _F, C.POINTER(_D), _D, C.POINTER(_D), _D]),
     "pdp3_ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
     "pdp3_ensemble_destroy": (_I, [_P]),
     "pdp3_ensemble_last_error": (C.c_char_p, [_P]),
@@ -279,6 +284,18 @@ class Simulation:
         self._chk(self.L.pdp3_gauss_residual(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
         return R, rho
 
+    def temperature(self, idx):
+        """TemperatureClassic of species idx on the device: (n_r, n_z) in eV"""
+        out = np.empty((self.nr, self.nz))
+        self._chk(self.L.pdp3_temperature(self.h, int(idx), _dp(out)))
+        return out
+
+    def density(self, idx):
+        """ParticlesDensity::calc_density of species idx on the device: (n_r, n_z)"""
+        out = np.empty((self.nr, self.nz))
+        self._chk(self.L.pdp3_density(self.h, int(idx), _dp(out)))
+        return out
+
     def adjoint_run(self, nsteps, checkpoint_every, window_steps, z_lo, z_hi):
         """(phi, [dphi/dw per species]) of the axis E_z^2 objective over the
         last window_steps of nsteps iterations (the simulation advances)."""
@@ -289,6 +306,38 @@ class Simulation:
         self._chk(self.L.pdp3_adjoint_run(self.h, int(nsteps), int(checkpoint_every), int(window_steps),
                                           float(z_lo), float(z_hi), C.byref(phi), arr))
         return phi.value, [o[:n] for o, n in zip(outs, counts)]
+
+    def tangent_run(self, nsteps, window_steps, z_lo, z_hi, dw):
+        """(phi, dphi[TANGENT_DIRS]) of the axis E_z^2 objective and its
+        directional derivatives along weight tangents dw: one (TANGENT_DIRS, n)
+        array (or None) per species (the simulation advances)."""
+        arrs = []
+        for j in range(self.n_species()):
+            a = None if dw is None or j >= len(dw) or dw[j] is None else \
+                np.ascontiguousarray(dw[j], dtype=np.float64).reshape(TANGENT_DIRS, self.count(j))
+            arrs.append(a)
+        ptrs = (_D * max(len(arrs), 1))(*[_dp(a) for a in arrs])
+        phi = C.c_double()
+        dphi = np.zeros(TANGENT_DIRS)
+        self._chk(self.L.pdp3_tangent_run(self.h, int(nsteps), int(window_steps), float(z_lo), float(z_hi), ptrs,
+                                          C.byref(phi), _dp(dphi)))
+        return phi.value, dphi
+
+    def adjoint_tangent_run(self, nsteps, checkpoint_every, window_steps, z_lo, z_hi, dw):
+        """adjoint_run plus the tangent_run derivatives along dw, both around
+        the trajectory of the adjoint's own forward sweep: (phi, [dphi/dw per
+        species], dphi[TANGENT_DIRS])"""
+        counts = [self.count(j) for j in range(self.n_species())]
+        arrs = [None if dw[j] is None else np.ascontiguousarray(dw[j], dtype=np.float64).reshape(TANGENT_DIRS, n)
+                for j, n in enumerate(counts)]
+        pdw = (_D * max(len(arrs), 1))(*[_dp(a) for a in arrs])
+        outs = [np.zeros(max(n, 1)) for n in counts]
+        pout = (_D * max(len(outs), 1))(*[_dp(o) for o in outs])
+        phi = C.c_double()
+        dphi = np.zeros(TANGENT_DIRS)
+        self._chk(self.L.pdp3_adjoint_tangent_run(self.h, int(nsteps), int(checkpoint_every), int(window_steps),
+                                                  float(z_lo), float(z_hi), pdw, C.byref(phi), pout, _dp(dphi)))
+        return phi.value, [o[:n] for o, n in zip(outs, counts)], dphi
 
     def objective_run(self, nsteps, window_steps, z_lo, z_hi):
         phi = C.c_double()

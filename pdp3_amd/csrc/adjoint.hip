@@ -29,6 +29,7 @@ namespace pdp3 {
 namespace {
 
 constexpr int kDN = 4;       // directions per pass
+static_assert(kDN == kTangentDirs, "the tangent-linear model runs one direction per dual slot");
 constexpr int kInputs = 12;  // r, z, v0, v1, v2, e0, e1, e2, h0, h1, h2, w
 constexpr int kPasses = kInputs / kDN;
 
@@ -327,7 +328,8 @@ __device__ void vay_d(D q, D m, double dt, D e[3], D b[3], D v[3]) {
 __device__ __forceinline__ D ring(D F, D dzx, D ra, D rb, double vol) { return F * kPi * dzx * (rb * rb - ra * ra) / vol; }
 
 // azimuthal_current_distribution, src/particles.cpp:688-784
-__device__ void jphi_d(const GridDesc &g, AdjSink &sk, D q, D r, D z, D vphi) {
+template <class S>
+__device__ void jphi_d(const GridDesc &g, S &sk, D q, D r, D z, D vphi) {
   const double dr = g.dr, dz = g.dz;
   int r_i = cell_number(r.v, dr), z_k = cell_number(z.v, dz);
   if (r_i < 0) r_i = 0;
@@ -371,7 +373,8 @@ __device__ __forceinline__ D m_of_d(D x) {
 // (and the polynomial forms on the axis cell).  The reference's forms are
 // differences of O(i) terms: their derivative loses ~1/x digits for nearly
 // axial paths, which a relativistic beam's particles are.
-__device__ void segment_d(const GridDesc &g, AdjSink &acc, D q, D radius_new, D longitude_new, D radius_old,
+template <class S>
+__device__ void segment_d(const GridDesc &g, S &acc, D q, D radius_new, D longitude_new, D radius_old,
                           D longitude_old, int i_n, int k_n) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
   const double delta_r = radius_new.v - radius_old.v;
@@ -452,7 +455,8 @@ __device__ void segment_d(const GridDesc &g, AdjSink &acc, D q, D radius_new, D 
 }
 
 // strict_motion_weighting, src/particles.cpp:786-988
-__device__ void strict_d(const GridDesc &g, AdjSink &acc, D q, D radius_new, D longitude_new, D radius_old,
+template <class S>
+__device__ void strict_d(const GridDesc &g, S &acc, D q, D radius_new, D longitude_new, D radius_old,
                          D longitude_old) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
   int i_n = cell_number(radius_new.v, dr), k_n = cell_number(longitude_new.v, dz);
@@ -540,7 +544,8 @@ struct SegD {
 };
 
 // current_distribution, src/particles.cpp:480-685
-__device__ void deposit_j_d(const GridDesc &g, AdjSink &acc, D q, D pr, D pz, D por, D poz) {
+template <class S>
+__device__ void deposit_j_d(const GridDesc &g, S &acc, D q, D pr, D pz, D por, D poz) {
   const double dr = g.dr, dz = g.dz;
   int i_n = cell_number(pr.v, dr), k_n = cell_number(pz.v, dz);
   int i_o = cell_number(por.v, dr), k_o = cell_number(poz.v, dz);
@@ -806,6 +811,165 @@ __global__ void __launch_bounds__(256) k_adj_particles(GridDesc g, AdjFields f, 
   }
 }
 
+// ------------------------------------------------ tangent-linear (forward mode)
+// The same particle step in dual numbers, seeded with kDN tangent directions
+// of the state, of the gathered fields and of the weight instead of unit
+// vectors: the derivative parts of the outputs are the tangents of the new
+// state and of every deposit (J-dot, scattered with atomics).  Together with
+// the forward field updates applied to tangent grids this is the
+// tangent-linear model of the loop, the exact transpose of what the adjoint
+// above computes; <dphi/dw, dw> from the adjoint must equal the tangent of phi
+// along dw to round-off (the dot-product test).
+
+struct TlmFields {  // tangents of E^n and Hh^n, one set per direction
+  const double *er[kDN], *ephi[kDN], *ez[kDN], *hhr[kDN], *hhphi[kDN], *hhz[kDN];
+};
+struct TlmJ {  // tangents of J^n, one set per direction
+  double *jr[kDN], *jphi[kDN], *jz[kDN];
+};
+struct TlmSpecies {
+  const double *r, *z, *vr, *vphi, *vz, *w;  // state at step n (values)
+  const uint8_t *alive;
+  double *tr, *tz, *tvr, *tvphi, *tvz;  // kDN x n tangents: in at step n, out at n+1
+  const double *tw;                     // kDN x n tangents of the weight
+  double charge, mass;
+  long long n;
+  int is_bunch;
+};
+
+// deposit tangents: every contribution v at node (i, k) adds its derivative
+// part (one per direction) to the tangent J grids
+struct TlmSink {
+  const GridDesc *g;
+  TlmJ j;
+  __device__ __forceinline__ void add(int comp, int i, int k, D v) {
+    if ((unsigned)i >= (unsigned)g->nr || (unsigned)k >= (unsigned)g->nz) return;
+    const size_t p = (size_t)i * (size_t)g->nz + (size_t)k;
+#pragma unroll
+    for (int d = 0; d < kDN; d++) {
+      double *J = comp == kAR ? j.jr[d] : comp == kAPhi ? j.jphi[d] : j.jz[d];
+      if (v.d[d] != 0.0) unsafeAtomicAdd(J + p, v.d[d]);
+    }
+  }
+};
+
+// the gather of the tangent fields at the particle's (value) position: the
+// gather is linear in the field, so its tangent along the field is the same
+// weights applied to the tangent grid
+__device__ __forceinline__ double gather_value(const GridDesc &g, const double *F, double r, double z, bool lo,
+                                               bool zhalf) {
+  GatherW gw = gather_w(g, cst(r), cst(z), lo, zhalf);
+  const double *row0 = F + (size_t)gw.ir * (size_t)g.nz + gw.kz;
+  const double *row1 = row0 + g.nz;
+  return gw.w[0].v * row0[0] + gw.w[1].v * row1[0] + gw.w[2].v * row0[1] + gw.w[3].v * row1[1];
+}
+
+__global__ void __launch_bounds__(256) k_tlm_particles(GridDesc g, AdjFields f, TlmFields tf, TlmJ tj,
+                                                       TlmSpecies s, int pusher) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.n) return;
+  const long long n = s.n;
+  if (s.alive[i] != 1) {
+    // dead (or padding) slots keep zero tangents; a particle injected later
+    // starts with zero tangents, as its injected state does not depend on w
+#pragma unroll
+    for (int d = 0; d < kDN; d++) s.tr[d * n + i] = s.tz[d * n + i] = s.tvr[d * n + i] = s.tvphi[d * n + i] =
+        s.tvz[d * n + i] = 0.0;
+    return;
+  }
+  auto dual = [&](double v, const double *t) {
+    D x;
+    x.v = v;
+#pragma unroll
+    for (int d = 0; d < kDN; d++) x.d[d] = t[d * n + i];
+    return x;
+  };
+  TlmSink sk{&g, tj};
+  D r = dual(s.r[i], s.tr), z = dual(s.z[i], s.tz);
+  D v[3] = {dual(s.vr[i], s.tvr), dual(s.vphi[i], s.tvphi), dual(s.vz[i], s.tvz)};
+  D w = dual(s.w[i], s.tw);
+  D q = s.charge * w, m = s.mass * w;
+  const D r_old = r, z_old = z;
+  const double r0v = r.v, z0v = z.v;
+  D e[3], h[3];
+  e[0] = gather_d(g, f.er, r, z, true, false);
+  e[1] = gather_d(g, f.ephi, r, z, false, false);
+  e[2] = gather_d(g, f.ez, r, z, false, true);
+  h[0] = gather_d(g, f.hhr, r, z, false, true);
+  h[1] = gather_d(g, f.hhphi, r, z, true, true);
+  h[2] = gather_d(g, f.hhz, r, z, true, false);
+#pragma unroll
+  for (int d = 0; d < kDN; d++) {
+    e[0].d[d] += gather_value(g, tf.er[d], r0v, z0v, true, false);
+    e[1].d[d] += gather_value(g, tf.ephi[d], r0v, z0v, false, false);
+    e[2].d[d] += gather_value(g, tf.ez[d], r0v, z0v, false, true);
+    h[0].d[d] += gather_value(g, tf.hhr[d], r0v, z0v, false, true);
+    h[1].d[d] += gather_value(g, tf.hhphi[d], r0v, z0v, true, true);
+    h[2].d[d] += gather_value(g, tf.hhz[d], r0v, z0v, true, false);
+  }
+  if (pusher == 1)
+    vay_d(q, m, g.dt, e, h, v);
+  else
+    boris_d(q, m, g.dt, e, h, v);
+  const double half_dt = g.dt / 2.;
+  D x = r + v[0] * half_dt;
+  D y = 0.0 + v[1] * half_dt;
+  z = z + v[2] * half_dt;
+  r = dsqrt(x * x + y * y);
+  jphi_d(g, sk, q, r, z, v[1]);
+  x = r + v[0] * half_dt;
+  y = 0.0 + v[1] * half_dt;
+  z = z + v[2] * half_dt;
+  bool alive = true;
+  walls_d(g, s.is_bunch != 0, x, z, v[0], v[2], alive);
+  D rr = dsqrt(x * x + y * y);
+  D st = y / rr;
+  r = rr;
+  if (alive) {
+    deposit_j_d(g, sk, q, r, z, r_old, z_old);
+    D ct = dsqrt(1 - st * st);
+    D u2 = ct * v[0] - st * v[1];
+    D w2 = st * v[0] + ct * v[1];
+    v[0] = u2;
+    v[1] = w2;
+  }
+#pragma unroll
+  for (int d = 0; d < kDN; d++) {
+    const bool keep = alive;
+    s.tr[d * n + i] = keep ? r.d[d] : 0.0;
+    s.tz[d * n + i] = keep ? z.d[d] : 0.0;
+    s.tvr[d * n + i] = keep ? v[0].d[d] : 0.0;
+    s.tvphi[d * n + i] = keep ? v[1].d[d] : 0.0;
+    s.tvz[d * n + i] = keep ? v[2].d[d] : 0.0;
+  }
+}
+
+// tangent of the objective: phi_dot[d] += scale * sum 2 E_z(0, k) E_z-dot_d(0, k)
+// over k0..k1, and phi += scale * sum E_z(0,k)^2 (one block)
+__global__ void __launch_bounds__(256) k_tlm_objective(GridDesc g, const double *ez, const double *ezd0,
+                                                       const double *ezd1, const double *ezd2, const double *ezd3,
+                                                       int k0, int k1, double scale, double *phi) {
+  __shared__ double part[5][256];
+  double acc[5] = {0, 0, 0, 0, 0};
+  const double *ezd[4] = {ezd0, ezd1, ezd2, ezd3};
+  for (int k = k0 + threadIdx.x; k <= k1; k += blockDim.x) {
+    const double e = ez[k];
+    acc[0] += scale * e * e;
+#pragma unroll
+    for (int d = 0; d < 4; d++) acc[1 + d] += scale * 2.0 * e * ezd[d][k];
+  }
+#pragma unroll
+  for (int a = 0; a < 5; a++) part[a][threadIdx.x] = acc[a];
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2)
+#pragma unroll
+      for (int a = 0; a < 5; a++) part[a][threadIdx.x] += part[a][threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) phi[threadIdx.x] += part[threadIdx.x][0];
+}
+
 // ------------------------------------------------------------- the fields
 
 struct AdjE {  // adjoints of E, H (after step), Hh, J and the coefficients
@@ -990,6 +1154,37 @@ void launch_adj_particles(const GridDesc &g, const double *const *fields6, doubl
 void launch_adj_seed(const GridDesc &g, const double *ez, double *lez, int k0, int k1, double scale, double *phi,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_adj_seed, dim3(1), dim3(256), 0, st, g, ez, lez, k0, k1, scale, phi);
+}
+
+void launch_tlm_particles(const GridDesc &g, const double *const *fields6, double *const *const *tgrid,
+                          const double *r, const double *z, const double *vr, const double *vphi, const double *vz,
+                          const double *w, const uint8_t *alive, double *const *tstate, const double *tw,
+                          double charge, double mass, long long n, int is_bunch, int pusher, hipStream_t st) {
+  if (n <= 0) return;
+  AdjFields f{fields6[0], fields6[1], fields6[2], fields6[3], fields6[4], fields6[5]};
+  TlmFields tf;
+  TlmJ tj;
+  for (int d = 0; d < kDN; d++) {
+    double *const *a = tgrid[d];
+    tf.er[d] = a[kER];
+    tf.ephi[d] = a[kEPhi];
+    tf.ez[d] = a[kEZ];
+    tf.hhr[d] = a[kHHR];
+    tf.hhphi[d] = a[kHHPhi];
+    tf.hhz[d] = a[kHHZ];
+    tj.jr[d] = a[kJR];
+    tj.jphi[d] = a[kJPhi];
+    tj.jz[d] = a[kJZ];
+  }
+  TlmSpecies s{r, z, vr, vphi, vz, w, alive, tstate[0], tstate[1], tstate[2], tstate[3], tstate[4], tw,
+               charge, mass, n, is_bunch};
+  hipLaunchKernelGGL(k_tlm_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, f, tf, tj, s, pusher);
+}
+
+void launch_tlm_objective(const GridDesc &g, const double *ez, double *const *const *tgrid, int k0, int k1,
+                          double scale, double *phi5, hipStream_t st) {
+  hipLaunchKernelGGL(k_tlm_objective, dim3(1), dim3(256), 0, st, g, ez, tgrid[0][kEZ], tgrid[1][kEZ],
+                     tgrid[2][kEZ], tgrid[3][kEZ], k0, k1, scale, phi5);
 }
 
 }  // namespace pdp3

@@ -310,6 +310,24 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
     if (hipMemcpy(c.rows, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
       return bail(PDP3_ERR_HIP);
   }
+  {
+    // the FP64 form's divisors: CELL_VOLUME and CYL_VOL in the reference's
+    // expression order (include/geometry.h:33-39), with IEEE reciprocals
+    const double dr = c.g.dr, dz = c.g.dz;
+    std::vector<double2> vol((size_t)n_grid_r + 2);
+    for (int i = 0; i < n_grid_r + 2; i++) {
+      const double v = kPi * dz * dr * dr * 2.0 * i;
+      vol[i] = make_double2(v, i ? 1.0 / v : 0.0);
+    }
+    c.g.inv_dr = 1.0 / dr;
+    c.g.inv_dz = 1.0 / dz;
+    c.g.cyl = kPi * dz * dr * dr / 4.;
+    c.g.inv_cyl = 1.0 / c.g.cyl;
+    if ((st = dalloc(ctx, &c.vol64, vol.size()))) return bail(st);
+    if (hipMemcpy(c.vol64, vol.data(), vol.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(PDP3_ERR_HIP);
+    c.g.vol = c.vol64;
+  }
   *out = ctx;
   return PDP3_OK;
 }
@@ -328,7 +346,9 @@ int pdp3_ctx_destroy(pdp3_ctx *ctx) {
   dfree(c.sort_ws);
   dfree(c.rho_all);
   dfree(c.gauss);
+  dfree(c.probe_ws);
   dfree(c.rows);
+  dfree(c.vol64);
   if (c.err_host) (void)hipHostFree(c.err_host);
   for (auto &t : c.timers) {
     for (auto &ev : t.pending) {
@@ -815,26 +835,71 @@ int pdp3_particles_charge_weighting(pdp3_ctx *ctx, int index) {
   return PDP3_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// the Gauss residual into device buffers rho and res (cells() values each):
+// rho cleared by a fill kernel, then the charge weighting of every species of
+// the loop (bunches once they are created), then the residual, which writes
+// every node
+int gauss_into(pdp3_ctx *ctx, double *rho, double *res, double *residual_out, double *rho_out) {
+  Ctx &c = ctx->c;
+  int st;
+  launch_fill(rho, c.cells(), 0.0, c.stream);
+  for (int j : c.order) {
+    if (is_f32(c))
+      launch_charge_weighting_f32(c, c.species[j], rho);
+    else
+      launch_charge_weighting_f64(c, c.species[j], rho);
+  }
+  launch_gauss_residual(c, rho, res);
+  CHECK_LAUNCH(ctx);
+  if ((st = check_device_errors(ctx))) return st;
+  const size_t b = c.cells() * sizeof(double);
+  if (residual_out) HIP_TRY(ctx, hipMemcpyAsync(residual_out, res, b, hipMemcpyDeviceToHost, c.stream));
+  if (rho_out) HIP_TRY(ctx, hipMemcpyAsync(rho_out, rho, b, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out) {
   if (!ctx) return PDP3_ERR_BAD_ARG;
   Ctx &c = ctx->c;
   int st;
   if (!c.rho_all && (st = dalloc(ctx, &c.rho_all, c.cells()))) return st;
   if (!c.gauss && (st = dalloc(ctx, &c.gauss, c.cells()))) return st;
-  launch_fill(c.rho_all, c.cells(), 0.0, c.stream);
-  // every species of the loop (bunches once they are created)
-  for (int j : c.order) {
-    if (is_f32(c))
-      launch_charge_weighting_f32(c, c.species[j], c.rho_all);
-    else
-      launch_charge_weighting_f64(c, c.species[j], c.rho_all);
-  }
-  launch_gauss_residual(c, c.rho_all, c.gauss);
+  return gauss_into(ctx, c.rho_all, c.gauss, residual_out, rho_out);
+}
+
+int pdp3_temperature(pdp3_ctx *ctx, int index, double *out) {
+  if (!valid_species(ctx, index) || !out) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  int st;
+  if (!c.probe_ws && (st = dalloc(ctx, &c.probe_ws, 6 * c.cells()))) return st;
+  double *res = c.probe_ws + 5 * c.cells();
+  if (!launch_temperature(c, c.species[index], c.probe_ws, res))
+    return fail(ctx, PDP3_ERR_STATE, "the species has no particle state yet");
   CHECK_LAUNCH(ctx);
   if ((st = check_device_errors(ctx))) return st;
-  const size_t b = c.cells() * sizeof(double);
-  if (residual_out) HIP_TRY(ctx, hipMemcpyAsync(residual_out, c.gauss, b, hipMemcpyDeviceToHost, c.stream));
-  if (rho_out) HIP_TRY(ctx, hipMemcpyAsync(rho_out, c.rho_all, b, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(out, res, c.cells() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return PDP3_OK;
+}
+
+int pdp3_density(pdp3_ctx *ctx, int index, double *out) {
+  if (!valid_species(ctx, index) || !out) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  int st;
+  if (!c.probe_ws && (st = dalloc(ctx, &c.probe_ws, 6 * c.cells()))) return st;
+  double *res = c.probe_ws + 5 * c.cells();
+  if (!launch_density(c, c.species[index], res))
+    return fail(ctx, PDP3_ERR_STATE, "the species has no particle state yet");
+  CHECK_LAUNCH(ctx);
+  if ((st = check_device_errors(ctx))) return st;
+  HIP_TRY(ctx, hipMemcpyAsync(out, res, c.cells() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   return PDP3_OK;
 }
@@ -1080,12 +1145,163 @@ struct SpeciesSnap {
 
 long long adj_count(const Ctx &c, const Species &sp) { return is_f32(c) ? sp.n : sp.n_cap; }
 
+// the tangent-linear model carried along a forward sweep of the loop:
+// kTangentDirs directions of the particle weights; its grids are advanced by
+// the context's own (linear) field updates through shadow contexts that
+// share the geometry, PML coefficients and stream; the particle tangents by
+// the dual-number step (launch_tlm_particles) right after each H update
+struct TangentSweep {
+  static constexpr int D = kTangentDirs;
+  pdp3_ctx *ctx = nullptr;
+  DBuf tgrid_mem, phi_dev;
+  std::vector<Ctx> shadow;
+  double *tg[D][kNComp] = {};
+  double *const *tgp[D] = {};
+  std::vector<DBuf> tstate;  // per species: r, z, vr, vphi, vz tangents and the weight tangents (D x n each)
+
+  int init(pdp3_ctx *c_, const double *const *dw) {
+    ctx = c_;
+    Ctx &c = ctx->c;
+    const size_t cells = c.cells();
+    const int nsp = (int)c.species.size();
+    HIP_TRY(ctx, hipMalloc(&tgrid_mem.p, (size_t)D * kNComp * cells * sizeof(double)));
+    HIP_TRY(ctx, hipMemsetAsync(tgrid_mem.p, 0, (size_t)D * kNComp * cells * sizeof(double), c.stream));
+    HIP_TRY(ctx, hipMalloc(&phi_dev.p, (1 + D) * sizeof(double)));
+    HIP_TRY(ctx, hipMemsetAsync(phi_dev.p, 0, (1 + D) * sizeof(double), c.stream));
+    shadow.assign(D, Ctx());
+    for (int d = 0; d < D; d++) {
+      Ctx &sh = shadow[d];
+      sh.g = c.g;
+      sh.stream = c.stream;
+      sh.koef_e = c.koef_e;
+      sh.koef_h = c.koef_h;
+      for (int k = 0; k < kNComp; k++)
+        sh.a[k] = tg[d][k] = tgrid_mem.as<double>() + ((size_t)d * kNComp + k) * cells;
+      tgp[d] = tg[d];
+    }
+    tstate = std::vector<DBuf>((size_t)nsp * 6);
+    for (int j = 0; j < nsp; j++) {
+      Species &sp = c.species[j];
+      const long long n = adj_count(c, sp);
+      const size_t b = (size_t)std::max<long long>(n, 1) * D * sizeof(double);
+      for (int a = 0; a < 6; a++) {
+        HIP_TRY(ctx, hipMalloc(&tstate[j * 6 + a].p, b));
+        HIP_TRY(ctx, hipMemsetAsync(tstate[j * 6 + a].p, 0, b, c.stream));
+      }
+      if (dw && dw[j] && n > 0) {
+        // logical order -> the index space of the adjoint (FP64 form: slots)
+        std::vector<double> h((size_t)n * D, 0.0);
+        for (int d = 0; d < D; d++)
+          for (long long p = 0; p < n; p++) {
+            const long long l = (!is_f32(c) && sp.chunked) ? logical_of(p) : p;
+            if (l < sp.n) h[(size_t)d * n + p] = dw[j][(size_t)d * sp.n + l];
+          }
+        HIP_TRY(ctx, hipMemcpyAsync(tstate[j * 6 + 5].p, h.data(), h.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, c.stream));
+        HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+      }
+    }
+    return PDP3_OK;
+  }
+  // after the H update of an iteration: E^n and Hh^n are final and the
+  // particles are still at step n
+  int after_h() {
+    Ctx &c = ctx->c;
+    const size_t cells = c.cells();
+    for (int d = 0; d < D; d++) {
+      launch_h_update(shadow[d]);
+      launch_fill(tg[d][kJR], 3 * cells, 0.0, c.stream);  // J_r, J_phi, J_z are contiguous
+    }
+    const double *f6[6] = {c.a[kER], c.a[kEPhi], c.a[kEZ], c.a[kHHR], c.a[kHHPhi], c.a[kHHZ]};
+    for (int j = 0; j < (int)c.species.size(); j++) {
+      Species &sp = c.species[j];
+      const long long cnt = adj_count(c, sp);
+      if (cnt <= 0) continue;
+      if (is_f32(c)) launch_f64_from_f32(c, sp);  // logical-order FP64 staging
+      double *ts[5];
+      for (int a = 0; a < 5; a++) ts[a] = tstate[j * 6 + a].as<double>();
+      launch_tlm_particles(c.g, f6, tgp, sp.f64.r, sp.f64.z, sp.f64.vr, sp.f64.vphi, sp.f64.vz, sp.f64.w,
+                           sp.f64.alive, ts, tstate[j * 6 + 5].as<double>(), sp.charge, sp.mass, cnt,
+                           sp.kind == PDP3_SPECIES_BUNCH, c.pusher, c.stream);
+    }
+    CHECK_LAUNCH(ctx);
+    return PDP3_OK;
+  }
+  // after the iteration (E^{n+1} final): the tangent E update and, inside the
+  // objective's window, its tangent
+  int after_step(bool in_window, int k0, int k1) {
+    Ctx &c = ctx->c;
+    for (int d = 0; d < D; d++) launch_e_update(shadow[d]);
+    if (in_window && k1 >= k0)
+      launch_tlm_objective(c.g, c.a[kEZ], tgp, k0, k1, c.g.dt * c.g.dz, phi_dev.as<double>(), c.stream);
+    CHECK_LAUNCH(ctx);
+    return PDP3_OK;
+  }
+  int result(double *phi_out, double *dphi_out) {
+    double out[1 + D];
+    HIP_TRY(ctx, hipMemcpy(out, phi_dev.p, sizeof(out), hipMemcpyDeviceToHost));
+    if (phi_out) *phi_out = out[0];
+    if (dphi_out)
+      for (int d = 0; d < D; d++) dphi_out[d] = out[1 + d];
+    return PDP3_OK;
+  }
+};
+
+// the axis window of the objective: E_z(0, k) sits at z = (k + 1/2) dz
+void objective_window(const Ctx &c, double z_lo, double z_hi, int &k0, int &k1) {
+  k0 = std::max((int)std::ceil(z_lo / c.g.dz - 0.5), 0);
+  k1 = std::min((int)std::floor(z_hi / c.g.dz - 0.5), c.g.nz - 2);
+}
+
+// the adjoint run, optionally carrying the tangent-linear model along its
+// first forward sweep (dw non-null): the tangents then follow exactly the
+// trajectory that the checkpoints record
+int adjoint_impl(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo, double z_hi,
+                 double *phi_out, double *const *dphi_dw, const double *const *dw, double *dphi_tangent);
+
 }  // namespace
 
 extern "C" {
 
 int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo, double z_hi,
                      double *phi_out, double *const *dphi_dw) {
+  return adjoint_impl(ctx, nsteps, checkpoint_every, window_steps, z_lo, z_hi, phi_out, dphi_dw, nullptr, nullptr);
+}
+
+int pdp3_adjoint_tangent_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo,
+                             double z_hi, const double *const *dw, double *phi_out, double *const *dphi_dw,
+                             double *dphi_tangent) {
+  if (!dw) return PDP3_ERR_BAD_ARG;
+  return adjoint_impl(ctx, nsteps, checkpoint_every, window_steps, z_lo, z_hi, phi_out, dphi_dw, dw, dphi_tangent);
+}
+
+int pdp3_tangent_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi,
+                     const double *const *dw, double *phi_out, double *dphi_out) {
+  if (!ctx || nsteps < 1 || window_steps < 0 || window_steps > nsteps || !(z_hi >= z_lo)) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (c.sort_interval != 0)
+    return fail(ctx, PDP3_ERR_STATE, "the tangent-linear run needs a fixed particle order (sort_interval 0)");
+  int st;
+  if ((st = check_device_errors(ctx))) return st;
+  int k0, k1;
+  objective_window(c, z_lo, z_hi, k0, k1);
+  TangentSweep tl;
+  if ((st = tl.init(ctx, dw))) return st;
+  std::function<int()> hook = [&]() { return tl.after_h(); };
+  for (int n = 0; n < nsteps; n++) {
+    if ((st = step_one(ctx, &hook))) return st;
+    if ((st = tl.after_step(n >= nsteps - window_steps, k0, k1))) return st;
+  }
+  if ((st = check_device_errors(ctx))) return st;
+  return tl.result(phi_out, dphi_out);
+}
+
+}  // extern "C"
+
+namespace {
+
+int adjoint_impl(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window_steps, double z_lo, double z_hi,
+                 double *phi_out, double *const *dphi_dw, const double *const *dw, double *dphi_tangent) {
   if (!ctx || nsteps < 1 || checkpoint_every < 1 || window_steps < 0 || window_steps > nsteps || !(z_hi >= z_lo))
     return PDP3_ERR_BAD_ARG;
   Ctx &c = ctx->c;
@@ -1094,21 +1310,24 @@ int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window
   if ((st = check_device_errors(ctx))) return st;
   const size_t cells = c.cells();
   const int nsp = (int)c.species.size();
-  // axis window of the objective: E_z(0, k) sits at z = (k + 1/2) dz
-  int k0 = (int)std::ceil(z_lo / c.g.dz - 0.5), k1 = (int)std::floor(z_hi / c.g.dz - 0.5);
-  k0 = std::max(k0, 0);
-  k1 = std::min(k1, c.g.nz - 2);
+  int k0, k1;
+  objective_window(c, z_lo, z_hi, k0, k1);
   const double scale = c.g.dt * c.g.dz;
 
-  // forward with checkpoints
+  // forward with checkpoints (and the tangents, when requested)
   const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
   std::vector<Checkpoint> ck(nck);
   Checkpoint final_state;
+  TangentSweep tl;
+  if (dw && (st = tl.init(ctx, dw))) return st;
+  std::function<int()> tl_hook = [&]() { return tl.after_h(); };
   for (int n = 0; n < nsteps; n++) {
     if (n % checkpoint_every == 0 && (st = ckpt_save(ctx, ck[n / checkpoint_every]))) return st;
-    if ((st = step_one(ctx, nullptr))) return st;
+    if ((st = step_one(ctx, dw ? &tl_hook : nullptr))) return st;
+    if (dw && (st = tl.after_step(n >= nsteps - window_steps, k0, k1))) return st;
   }
   if ((st = check_device_errors(ctx)) || (st = ckpt_save(ctx, final_state))) return st;
+  if (dw && (st = tl.result(nullptr, dphi_tangent))) return st;
 
   // adjoint state
   DBuf lam_grid, phi_dev, ez_rows;
@@ -1222,6 +1441,10 @@ int pdp3_adjoint_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, int window
   }
   return ckpt_restore(ctx, final_state);
 }
+
+}  // namespace
+
+extern "C" {
 
 int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi, double *phi_out) {
   if (!ctx || nsteps < 0 || window_steps < 0 || window_steps > nsteps || !(z_hi >= z_lo) || !phi_out)

@@ -35,6 +35,13 @@ struct GridDesc {
   double dr, dz;        // src/geometry.cpp:45-53
   double r_size, z_size;
   double dt;
+  // the FP64 form's shared divisors, correctly rounded on the host: 1/dr,
+  // 1/dz, CYL_VOL(dz, dr) and its reciprocal, and per radial index
+  // i = 0 .. nr+1 the pair {CELL_VOLUME(i, dr, dz), 1/CELL_VOLUME(i, dr, dz)}
+  // (include/geometry.h:33-39), built at context creation
+  double inv_dr, inv_dz;
+  double cyl, inv_cyl;
+  const double2 *vol;
 };
 
 // Physical layout of plasma species.  Particles are processed by wavefronts
@@ -149,10 +156,15 @@ struct Ctx {
   // Gauss-law diagnostic: charge density of all species and the residual
   double *rho_all = nullptr;
   double *gauss = nullptr;
+  // probe workspace: 5 per-cell sums of the temperature probe, then the
+  // probe output (6 x cells doubles, allocated by the first probe)
+  double *probe_ws = nullptr;
   // FP32 form: per radial cell index i (0 .. nr-1) two float4 entries,
   // {1/(2i), 1/(2(i+1)), 1/(2(i+2)), 1/(2(i+3))} and {1/i, 1/(i+1), 1/(i+1/2),
   // i(i+1)}, with 1/0 stored as 0 (built at context creation)
   float4 *rows = nullptr;
+  // FP64 form: the cell-volume table GridDesc::vol points to
+  double2 *vol64 = nullptr;
   size_t cells() const { return (size_t)g.nr * (size_t)g.nz; }
 };
 
@@ -189,6 +201,19 @@ void launch_adj_particles(const GridDesc &g, const double *const *fields6, doubl
                           long long n, int is_bunch, int pusher, hipStream_t st);
 void launch_adj_seed(const GridDesc &g, const double *ez, double *lez, int k0, int k1, double scale, double *phi,
                      hipStream_t st);
+// tangent-linear model of one iteration's particle step (kTangentDirs
+// directions at once): tgrid[d][comp] are direction d's tangent grids (E, Hh
+// read, J accumulated); tstate = 5 arrays (r, z, vr, vphi, vz) of
+// kTangentDirs x n tangents, updated in place; tw = kTangentDirs x n weight
+// tangents
+constexpr int kTangentDirs = 4;
+void launch_tlm_particles(const GridDesc &g, const double *const *fields6, double *const *const *tgrid,
+                          const double *r, const double *z, const double *vr, const double *vphi, const double *vz,
+                          const double *w, const uint8_t *alive, double *const *tstate, const double *tw,
+                          double charge, double mass, long long n, int is_bunch, int pusher, hipStream_t st);
+// phi5[0] += scale sum E_z(0,k)^2, phi5[1 + d] += scale sum 2 E_z E_z-dot_d over k0..k1
+void launch_tlm_objective(const GridDesc &g, const double *ez, double *const *const *tgrid, int k0, int k1,
+                          double scale, double *phi5, hipStream_t st);
 
 // FP64 plain form
 void launch_fused_f64(Ctx &c, Species &sp);
@@ -205,6 +230,12 @@ void launch_charge_weighting_f32(Ctx &c, Species &sp, double *rho = nullptr);
 void launch_bunch_inject_f32(Ctx &c, Species &sp, unsigned start, unsigned count, double dl);
 bool launch_f32_from_f64(Ctx &c, Species &sp);  // pack absolute FP64 state into cell form
 void launch_f64_from_f32(Ctx &c, Species &sp);  // unpack for download
+
+// probes of one species from the resident state (either form):
+// TemperatureClassic into out (sums: 5 x cells workspace) and
+// ParticlesDensity into out; false when the species has no state
+bool launch_temperature(Ctx &c, Species &sp, double *sums, double *out);
+bool launch_density(Ctx &c, Species &sp, double *out);
 
 // synthetic loader (writes the FP64 arrays; to_phys: in the chunked layout)
 void launch_init_uniform(Ctx &c, Species &sp, int species_index, uint64_t seed, int ppc,

@@ -42,4 +42,21 @@ __device__ __forceinline__ double rsqrt_cr(double x) {
   return best;
 }
 
+// x / y, correctly rounded, from yr = 1/y correctly rounded: q = x yr lies
+// within 1.5 ulp of x/y; one fma correction with the exact remainder x - q y
+// makes it faithful, and a second one correctly rounded (Markstein's theorem:
+// a faithful quotient corrected with the correctly rounded reciprocal is the
+// correctly rounded quotient).  Five FP64 operations instead of the IEEE
+// division sequence with its quarter-rate reciprocal, for divisors whose
+// reciprocal is shared (uniform or tabulated); valid for finite operands
+// without overflow or underflow, which is what the particle step divides.
+// A zero x yields +0 (the IEEE division keeps the sign of a -0 dividend).
+__device__ __forceinline__ double div_cr(double x, double y, double yr) {
+  double q = x * yr;
+  double r = fma(-q, y, x);
+  q = fma(r, yr, q);
+  r = fma(-q, y, x);
+  return fma(r, yr, q);
+}
+
 }  // namespace pdp3

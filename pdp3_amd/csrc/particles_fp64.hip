@@ -27,17 +27,28 @@ struct SpeciesArgs {
   int is_bunch;
 };
 
-// CELL_NUMBER, include/geometry.h:44: x in (i dx, (i+1) dx] -> i
-__device__ __forceinline__ int cell_number(double x, double dx) { return (int)ceil(x / dx) - 1; }
-// CELL_VOLUME and CYL_VOL, include/geometry.h:33-41
-__device__ __forceinline__ double cell_volume(int i, double dr, double dz) {
-  return kPi * dz * dr * dr * 2.0 * i;
+// CELL_NUMBER, include/geometry.h:44: x in (i dx, (i+1) dx] -> i; x / dx
+// correctly rounded from the context's 1/dr, 1/dz (div_cr)
+__device__ __forceinline__ int cell_r(const GridDesc &g, double x) { return (int)ceil(div_cr(x, g.dr, g.inv_dr)) - 1; }
+__device__ __forceinline__ int cell_z(const GridDesc &g, double x) { return (int)ceil(div_cr(x, g.dz, g.inv_dz)) - 1; }
+// CELL_VOLUME(i, dr, dz) and CYL_VOL(dz, dr) (include/geometry.h:33-41) with
+// their reciprocals, from the context's table (computed there in the
+// reference's expression order)
+struct Vol {
+  double v, inv;
+};
+__device__ __forceinline__ Vol vol_cell(const GridDesc &g, int i) {
+  const double2 t = g.vol[i];
+  return Vol{t.x, t.y};
 }
-__device__ __forceinline__ double cyl_vol(double z, double r) { return kPi * z * r * r / 4.; }
+__device__ __forceinline__ Vol vol_cyl(const GridDesc &g) { return Vol{g.cyl, g.inv_cyl}; }
 // `F * CYL_RNG_VOL(dzx, ra, rb) / vol` as the reference's macro expands
-__device__ __forceinline__ double ring_term(double F, double dzx, double ra, double rb, double vol) {
-  return F * kPi * dzx * (rb * rb - ra * ra) / vol;
+__device__ __forceinline__ double ring_term(double F, double dzx, double ra, double rb, Vol vol) {
+  return div_cr(F * kPi * dzx * (rb * rb - ra * ra), vol.v, vol.inv);
 }
+// sq / c^2 correctly rounded (the divisor of lib::get_gamma / get_gamma_inv)
+constexpr double kC2 = kLightSpeed * kLightSpeed;
+constexpr double kInvC2 = 1.0 / kC2;
 
 // four-node ring-volume gather shared by EField::get_field
 // (src/eField.cpp:204-316) and HField::get_field (src/hField.cpp:159-267).
@@ -49,8 +60,8 @@ __device__ double gather4(const GridDesc &g, const double *__restrict__ F, doubl
   const double dr = g.dr, dz = g.dz;
   double r1 = radius - 0.5 * dr;
   double r3 = radius + 0.5 * dr;
-  int i_r = lo ? cell_number(radius - 0.5 * dr, dr) : cell_number(radius, dr);
-  int k_z = zhalf ? cell_number(longitude - 0.5 * dz, dz) : cell_number(longitude, dz);
+  int i_r = lo ? cell_r(g, radius - 0.5 * dr) : cell_r(g, radius);
+  int k_z = zhalf ? cell_z(g, longitude - 0.5 * dz) : cell_z(g, longitude);
   if (i_r < 0) i_r = 0;
   if (k_z < 0) k_z = 0;
   if (i_r > g.nr - 2 || k_z > g.nz - 2) {
@@ -58,15 +69,16 @@ __device__ double gather4(const GridDesc &g, const double *__restrict__ F, doubl
     i_r = min(i_r, g.nr - 2);
     k_z = min(k_z, g.nz - 2);
   }
-  double vol_1, vol_2, r2, dz1, dz2;
+  Vol vol_1, vol_2;
+  double r2, dz1, dz2;
   if (lo) {
-    vol_1 = cell_volume(i_r + 1, dr, dz);
-    vol_2 = cell_volume(i_r + 3, dr, dz);
+    vol_1 = vol_cell(g, i_r + 1);
+    vol_2 = vol_cell(g, i_r + 3);
     r2 = (i_r + 1) * dr;
   } else {
-    vol_1 = (radius > dr) ? cell_volume(i_r, dr, dz) : cyl_vol(dz, dr);
+    vol_1 = (radius > dr) ? vol_cell(g, i_r) : vol_cyl(g);
     r2 = (i_r + 0.5) * dr;
-    vol_2 = cell_volume(i_r + 2, dr, dz);
+    vol_2 = vol_cell(g, i_r + 2);
   }
   if (zhalf) {
     dz1 = (k_z + 1.5) * dz - longitude;
@@ -108,7 +120,7 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) {
 
 // lib::get_gamma / get_gamma_inv, src/lib.cpp:45-80
 __device__ __forceinline__ bool gamma_of(double sq_velocity, double &gamma) {
-  double beta = sq_velocity / (kLightSpeed * kLightSpeed);
+  double beta = div_cr(sq_velocity, kC2, kInvC2);
   if (beta > 1) return false;
   double gm = 1 / sqrt(1.0 - beta);
   gamma = isinf(gm) ? 1e100 : gm;
@@ -116,7 +128,7 @@ __device__ __forceinline__ bool gamma_of(double sq_velocity, double &gamma) {
 }
 // pow(y, -0.5) as glibc returns it: the correctly rounded 1/sqrt(y)
 __device__ __forceinline__ double gamma_inv(double sq_velocity) {
-  return rsqrt_cr(1.0 + sq_velocity / (kLightSpeed * kLightSpeed));
+  return rsqrt_cr(1.0 + div_cr(sq_velocity, kC2, kInvC2));
 }
 
 // relativistic Boris, src/particles.cpp:1025-1150 (velocity part)
@@ -187,16 +199,16 @@ __device__ bool vay(double q, double m, double dt, double e[3], double b[3], dou
 template <class S>
 __device__ void deposit_jphi(const GridDesc &g, S &sk, double q, double r, double z, double vphi) {
   const double dr = g.dr, dz = g.dz;
-  int r_i = cell_number(r, dr);
-  int z_k = cell_number(z, dz);
+  int r_i = cell_r(g, r);
+  int z_k = cell_z(g, z);
   if (r_i < 0) r_i = 0;
   if (z_k < 0) z_k = 0;
   double r1 = r - 0.5 * dr;
   double r2 = (r_i + 0.5) * dr;
   double r3 = r + 0.5 * dr;
   double ro_v = q / (2. * kPi * dz * dr * r);
-  double v_1 = (r > dr) ? cell_volume(r_i, dr, dz) : cyl_vol(dz, dr);
-  double v_2 = cell_volume(r_i + 1, dr, dz);
+  Vol v_1 = (r > dr) ? vol_cell(g, r_i) : vol_cyl(g);
+  Vol v_2 = vol_cell(g, r_i + 1);
   double dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
   double dz2 = (z + 0.5 * dz) - (z_k + 0.5) * dz;
   sk.template add<kCPhi>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1) * vphi);
@@ -209,8 +221,8 @@ __device__ void deposit_jphi(const GridDesc &g, S &sk, double q, double r, doubl
 template <class S>
 __device__ void deposit_rho(const GridDesc &g, S &sk, double q, double r, double z) {
   const double dr = g.dr, dz = g.dz;
-  int r_i = cell_number(r, dr);
-  int z_k = cell_number(z, dz);
+  int r_i = cell_r(g, r);
+  int z_k = cell_z(g, z);
   if (r_i < 0) r_i = 0;
   if (z_k < 0) z_k = 0;
   double dz1 = (z_k + 0.5) * dz - (z - 0.5 * dz);
@@ -218,7 +230,7 @@ __device__ void deposit_rho(const GridDesc &g, S &sk, double q, double r, double
   if (r > dr) {
     double r1 = r - 0.5 * dr, r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (2. * kPi * dz * dr * r);
-    double v_1 = cell_volume(r_i, dr, dz), v_2 = cell_volume(r_i + 1, dr, dz);
+    Vol v_1 = vol_cell(g, r_i), v_2 = vol_cell(g, r_i + 1);
     sk.template add<kCRho>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1));
     sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
     sk.template add<kCRho>(r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1));
@@ -227,15 +239,15 @@ __device__ void deposit_rho(const GridDesc &g, S &sk, double q, double r, double
     r_i = 0;
     double r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (kPi * dz * (2. * r * r + dr * dr / 2.));
-    double v_1 = cyl_vol(dz, dr), v_2 = cell_volume(r_i + 1, dr, dz);
-    sk.template add<kCRho>(r_i, z_k, ro_v * kPi * dz1 * (dr * dr / 2. - r * dr + r * r) / v_1);
+    Vol v_1 = vol_cyl(g), v_2 = vol_cell(g, r_i + 1);
+    sk.template add<kCRho>(r_i, z_k, div_cr(ro_v * kPi * dz1 * (dr * dr / 2. - r * dr + r * r), v_1.v, v_1.inv));
     sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
-    sk.template add<kCRho>(r_i, z_k + 1, ro_v * kPi * dz2 * (dr * dr / 2. - r * dr + r * r) / v_1);
+    sk.template add<kCRho>(r_i, z_k + 1, div_cr(ro_v * kPi * dz2 * (dr * dr / 2. - r * dr + r * r), v_1.v, v_1.inv));
     sk.template add<kCRho>(r_i + 1, z_k + 1, ring_term(ro_v, dz2, r2, r3, v_2));
   } else {
     double r1 = r - 0.5 * dr, r2 = (r_i + 0.5) * dr, r3 = r + 0.5 * dr;
     double ro_v = q / (2. * kPi * dz * dr * r);
-    double v_1 = cyl_vol(dz, dr), v_2 = cell_volume(r_i + 1, dr, dz);
+    Vol v_1 = vol_cyl(g), v_2 = vol_cell(g, r_i + 1);
     sk.template add<kCRho>(r_i, z_k, ring_term(ro_v, dz1, r1, r2, v_1));
     sk.template add<kCRho>(r_i + 1, z_k, ring_term(ro_v, dz1, r2, r3, v_2));
     sk.template add<kCRho>(r_i, z_k + 1, ring_term(ro_v, dz2, r1, r2, v_1));
@@ -307,10 +319,10 @@ template <class S>
 __device__ void strict_motion(const GridDesc &g, S &acc, double q, double radius_new,
                               double longitude_new, double radius_old, double longitude_old) {
   const double dr = g.dr, dz = g.dz, delta_t = g.dt;
-  int i_n = cell_number(radius_new, dr);
-  int k_n = cell_number(longitude_new, dz);
-  int i_o = cell_number(radius_old, dr);
-  int k_o = cell_number(longitude_old, dz);
+  int i_n = cell_r(g, radius_new);
+  int k_n = cell_z(g, longitude_new);
+  int i_o = cell_r(g, radius_old);
+  int k_o = cell_z(g, longitude_old);
   if (i_n < 0) i_n = 0;
   if (k_n < 0) k_n = 0;
   if (i_o < 0) i_o = 0;
@@ -413,10 +425,10 @@ template <class S>
 __device__ void deposit_j(const GridDesc &g, S &acc, double q, double pr, double pz, double por,
                           double poz) {
   const double dr = g.dr, dz = g.dz;
-  int i_n = cell_number(pr, dr);
-  int k_n = cell_number(pz, dz);
-  int i_o = cell_number(por, dr);
-  int k_o = cell_number(poz, dz);
+  int i_n = cell_r(g, pr);
+  int k_n = cell_z(g, pz);
+  int i_o = cell_r(g, por);
+  int k_o = cell_z(g, poz);
   if (i_n < 0) i_n = 0;
   if (k_n < 0) k_n = 0;
   if (i_o < 0) i_o = 0;
@@ -648,7 +660,7 @@ __device__ __forceinline__ void step_particle(const GridDesc &g, const Fields &f
   const double q = s.charge * w, m = s.mass * w;
   const double r_old = r, z_old = z;
   // contributions of this particle are centred on its starting cell
-  int ci = cell_number(r, g.dr), ck = cell_number(z, g.dz);
+  int ci = cell_r(g, r), ck = cell_z(g, z);
   sk.begin_cell(ci < 0 ? 0 : ci, ck < 0 ? 0 : ck);
   double sin_t;
   if (!push_half(g, f, pusher, err, q, m, r, z, v, sin_t)) return;
@@ -664,17 +676,19 @@ __device__ __forceinline__ void step_particle(const GridDesc &g, const Fields &f
 
 // chunked plasma species: 4 wavefronts per block, each walking one chunk of
 // kChunk slots kPPT particles per lane, deposits through the tile sink
-// 3 waves per SIMD (168 VGPRs, a few spilled) measured 5% faster at cfg3 than
-// the 189 VGPRs / 2 waves the compiler picks by itself; 4 waves spill heavily
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// with the divisions taken from shared reciprocals (div_cr) the compiler's own
+// choice, 227 VGPRs at 2 waves per SIMD without spills, is the fastest: 25.9 ms
+// at cfg3 against 26.7 ms for the IEEE divisions at 3 waves (168 VGPRs, 12
+// spilled); forcing 3 waves with div_cr spills 146 VGPRs (43 ms)
+__global__ void __launch_bounds__(256)
 k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s, long long n_cap, int pusher) {
   __shared__ double box[TileSink::kSize];
   const long long base = (long long)blockIdx.x * kBlockParticles;
   // anchor: the block's first logical particle (smallest cell of a sorted block)
   int ai = 0, ak = 0;
   if (base < n_cap && s.alive[base] == 1) {
-    ai = max(0, cell_number(s.r[base], g.dr));
-    ak = max(0, cell_number(s.z[base], g.dz));
+    ai = max(0, cell_r(g, s.r[base]));
+    ak = max(0, cell_z(g, s.z[base]));
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);
@@ -705,8 +719,8 @@ __global__ void __launch_bounds__(256) k_fused_bunch(GridDesc g, Fields f, Globa
   for (int j = 0; j < PPT; j++) {
     long long p = cbase + (long long)j * 64;
     if (p < n_cap && s.alive[p] == 1) {
-      ai = min(ai, max(0, cell_number(s.r[p], g.dr)));
-      ak = min(ak, max(0, cell_number(s.z[p], g.dz)));
+      ai = min(ai, max(0, cell_r(g, s.r[p])));
+      ak = min(ak, max(0, cell_z(g, s.z[p])));
     }
   }
   block_min2(ai, ak);

@@ -61,6 +61,8 @@ void LoadInitParam::init_fields() {
   hfield->set_homogeneous_h(0., 0., 0.);
   c_current = std::make_unique<Current>(geom.get());
   c_rho_bunch = std::make_unique<ChargeDensity>(geom.get());
+  temperature = std::make_unique<TemperatureClassic>(geom.get());
+  density = std::make_unique<ParticlesDensity>(geom.get());
 }
 
 // src/loadInitParam.cpp:138-166
@@ -149,11 +151,21 @@ void LoadInitParam::dump_data(double step_time) {
       }
       w.write(file_name, it->second.data(), (size_t)geom->n_grid_z);
     } else if (w.spec.component == "T") {
+      // src/loadInitParam.cpp:369-379, on the device (no particle download)
       for (auto &p : species)
         if (p->name == w.spec.specie) {
-          p->download();
-          std::vector<double> t = temperature_classic(*geom, *p);
-          w.write(file_name, t.data(), (size_t)geom->n_grid_z - 1);
+          temperature->reset();
+          temperature->calc_t(p.get());
+          temperature->normalize(p.get());
+          w.write(file_name, temperature->t.data(), (size_t)geom->n_grid_z);
+        }
+    } else if (w.spec.component == "density") {
+      // src/loadInitParam.cpp:381-390, on the device
+      for (auto &p : species)
+        if (p->name == w.spec.specie) {
+          density->reset();
+          density->calc_density(p.get());
+          w.write(file_name, density->density.data(), (size_t)geom->n_grid_z);
         }
     } else {
       std::cerr << "WARNING! probe for component ``" << w.spec.component << "'' does not exist. Skipping"

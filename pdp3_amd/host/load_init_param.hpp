@@ -30,6 +30,8 @@ class LoadInitParam {
   std::unique_ptr<HField> hfield;
   std::unique_ptr<Current> c_current;
   std::unique_ptr<ChargeDensity> c_rho_bunch;
+  std::unique_ptr<TemperatureClassic> temperature;  // the "T" probes (device)
+  std::unique_ptr<ParticlesDensity> density;        // the "density" probes (device)
   std::vector<std::unique_ptr<Particles>> species;  // plasma species, then bunches
   ParticlesList p_list;
   std::vector<ProbePlain> c_probes;

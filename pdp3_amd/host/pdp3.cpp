@@ -1,5 +1,6 @@
 #include "pdp3.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -321,42 +322,23 @@ void ParticlesList::full_current_distribution(Current *c, Time *t) {
 
 // ------------------------------------------------------------- diagnostics
 
-std::vector<double> temperature_classic(const Geometry &g, const Particles &p) {
-  const int nr = g.n_grid_r, nz = g.n_grid_z;
-  const size_t cells = (size_t)nr * (size_t)nz;
-  std::vector<double> sum(cells, 0.0), vr(cells, 0.0), vphi(cells, 0.0), vz(cells, 0.0), count(cells, 0.0);
-  for (size_t j = 0; j < p.pos_r.size(); j++) {
-    if (!std::isfinite(p.pos_r[j]) || !std::isfinite(p.pos_z[j])) continue;
-    // CELL_NUMBER, include/geometry.h:44 (no clamp here, as the reference)
-    int i = (int)std::ceil(p.pos_r[j] / g.dr) - 1;
-    int k = (int)std::ceil(p.pos_z[j] / g.dz) - 1;
-    if (i < 0 || i >= nr || k < 0 || k >= nz) continue;
-    const size_t c = (size_t)i * nz + k;
-    const double a = p.vel_r[j], b = p.vel_phi[j], e = p.vel_z[j];
-    sum[c] += std::sqrt(a * a + b * b + e * e);
-    vr[c] += a;
-    vphi[c] += b;
-    vz[c] += e;
-    count[c] += 1;
-  }
-  const double rel_limit = 5e7;  // REL_LIMIT, src/temperatureClassic.cpp:4-6
-  std::vector<double> t((size_t)(nr - 1) * (size_t)(nz - 1));
-  for (int i = 0; i < nr - 1; i++)
-    for (int k = 0; k < nz - 1; k++) {
-      const size_t c = (size_t)i * nz + k;
-      double vec2 = vr[c] * vr[c] + vphi[c] * vphi[c] + vz[c] * vz[c];
-      double v2 = (sum[c] * sum[c] - vec2) / (count[c] * count[c]);
-      double val;
-      if (v2 < rel_limit * rel_limit) {
-        val = p.mass * v2 / 2;
-      } else {
-        // lib::get_gamma, src/lib.cpp:45-70
-        double gamma = 1 / std::sqrt(1.0 - v2 / kLightSpeed2);
-        val = p.mass * kLightSpeed2 * (gamma - 1);
-      }
-      t[(size_t)i * (nz - 1) + k] = val / std::fabs(p.charge);
-    }
-  return t;
+TemperatureClassic::TemperatureClassic(Geometry *g) : geom(g), t((size_t)g->n_grid_r * (size_t)g->n_grid_z, 0.0) {}
+
+void TemperatureClassic::reset() { std::fill(t.begin(), t.end(), 0.0); }
+
+void TemperatureClassic::calc_t(Particles *prtls) {
+  Device &d = device_of(geom);
+  d.check(pdp3_temperature(d.ctx(), prtls->index(), t.data()));
+}
+
+ParticlesDensity::ParticlesDensity(Geometry *g)
+    : geom(g), density((size_t)g->n_grid_r * (size_t)g->n_grid_z, 0.0) {}
+
+void ParticlesDensity::reset() { std::fill(density.begin(), density.end(), 0.0); }
+
+void ParticlesDensity::calc_density(Particles *prtls) {
+  Device &d = device_of(geom);
+  d.check(pdp3_density(d.ctx(), prtls->index(), density.data()));
 }
 
 }  // namespace pdp3

@@ -237,9 +237,32 @@ class BoundaryMaxwellConditions {
   void specify_initial_field(Geometry *cyl_geom, double E_fi_upper, double E_fi_left, double E_fi_right);
 };
 
-// TemperatureClassic, src/temperatureClassic.cpp:57-158, on the host mirrors
-// of a species: (n_grid_r-1) x (n_grid_z-1) temperatures in eV (NaN where a
-// cell holds no particle)
-std::vector<double> temperature_classic(const Geometry &g, const Particles &p);
+// TemperatureClassic, include/temperatureClassic.h: the per-cell sums and the
+// normalisation run on the device over the resident particle state
+// (pdp3_temperature); t is the host mirror, n_grid_r x n_grid_z row-major
+// (row n_grid_r-1 and column n_grid_z-1 are 0, empty cells NaN).
+class TemperatureClassic {
+ public:
+  Geometry *geom;
+  std::vector<double> t;
+  explicit TemperatureClassic(Geometry *g);
+  void reset();
+  // src/temperatureClassic.cpp:57-103 and :127-158 in one device pass
+  void calc_t(Particles *prtls);
+  // the device's calc_t already normalised: kept for the reference's call sequence
+  void normalize(Particles *prtls) {}
+};
+
+// ParticlesDensity, include/particlesDensity.h: calc_density on the device
+// (pdp3_density); density is the host mirror, n_grid_r x n_grid_z row-major.
+class ParticlesDensity {
+ public:
+  Geometry *geom;
+  std::vector<double> density;
+  explicit ParticlesDensity(Geometry *g);
+  void reset();
+  // src/particlesDensity.cpp:40-225
+  void calc_density(Particles *prtls);
+};
 
 }  // namespace pdp3

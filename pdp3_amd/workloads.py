@@ -127,26 +127,31 @@ def gaussian_bunch(rng, n, sigma_r, sigma_z, peak_density, velocity, dr, dz, cha
     return Bunch(charge, mass, r, w, u, duration, 0.0, velocity)
 
 
-def config(name: str, seed: int = DEFAULT_SEED, **over) -> Workload:
+def config(name: str, seed: int = DEFAULT_SEED, bunch_macro: int | None = None, ramp_cells: int | None = None,
+           **over) -> Workload:
     """The benchmark configurations.  cfg1: 64x256 CPU-sized case; cfg2:
     256x2048 production size with mobile ions and Vay; cfg3: 512x8192 at
-    100 ppc with a density ramp (the precision-form comparison)."""
+    100 ppc with a density ramp (the precision-form comparison).
+    bunch_macro and ramp_cells override the bunch's macro-particle count and
+    the density ramp's length in z-cells (reduced copies for parity tests);
+    other keywords override Workload fields (cells_r, cells_z, pml_cells,
+    pusher, steps)."""
     rng = np.random.default_rng(seed)
     if name == "cfg1":
         dr = dz = 5e-4
         w = Workload("cfg1", 64, 256, dr, dz, 500, 8, 0, seed=seed)
         w.plasma = [Plasma(-1, 1.0, 10, 1e17, 1.0)]
-        w.bunches = [uniform_bunch(rng, 1000, 2e-3, 5e-3, 1e16, 0.9 * LIGHT_SPEED, dr, dz)]
+        w.bunches = [uniform_bunch(rng, bunch_macro or 1000, 2e-3, 5e-3, 1e16, 0.9 * LIGHT_SPEED, dr, dz)]
     elif name == "cfg2":
         dr = dz = 2.5e-4
         w = Workload("cfg2", 256, 2048, dr, dz, 2000, 16, 1, seed=seed)
         w.plasma = [Plasma(-1, 1.0, 40, 1e17, 1.0), Plasma(1, 1836.0, 40, 1e17, 0.1)]
-        w.bunches = [gaussian_bunch(rng, 100000, 1e-3, 2e-3, 2e16, 0.99 * LIGHT_SPEED, dr, dz)]
+        w.bunches = [gaussian_bunch(rng, bunch_macro or 100000, 1e-3, 2e-3, 2e16, 0.99 * LIGHT_SPEED, dr, dz)]
     elif name == "cfg3":
         dr = dz = 2.5e-4
         w = Workload("cfg3", 512, 8192, dr, dz, 1000, 16, 0, seed=seed)
-        w.plasma = [Plasma(-1, 1.0, 100, 1e18, 1.0, ramp_cells=1000)]
-        w.bunches = [gaussian_bunch(rng, 1000000, 0.5e-3, 1e-3, 5e17, 0.999 * LIGHT_SPEED, dr, dz)]
+        w.plasma = [Plasma(-1, 1.0, 100, 1e18, 1.0, ramp_cells=1000 if ramp_cells is None else ramp_cells)]
+        w.bunches = [gaussian_bunch(rng, bunch_macro or 1000000, 0.5e-3, 1e-3, 5e17, 0.999 * LIGHT_SPEED, dr, dz)]
     else:
         raise ValueError(name)
     for k, v in over.items():

@@ -43,6 +43,23 @@ def golden(comp):
     return data.reshape(r1 - r0, z1 - z0)
 
 
+# temperature probes: (r_start, z_start, r_end, z_end) and species names
+T_FRAME = (100, 100, 150, 150)
+
+
+def golden_t(specie):
+    r0, z0, r1, z1 = T_FRAME
+    fn = os.path.join(GOLDEN, f"T__{specie}__frame_{r0}_{z0}_{r1}_{z1}__{FRAME_NO}.dat.gz")
+    with gzip.open(fn, "rb") as f:
+        data = np.array([float(x) for x in f.read().split()], dtype=np.float64)
+    return data.reshape(r1 - r0, z1 - z0)
+
+
+def t_frame(arr):
+    r0, z0, r1, z1 = T_FRAME
+    return arr[r0:r1, z0:z1]
+
+
 def dr_dz():
     return (GEOM["r_size"] / (GEOM["n_grid_r"] - 1), GEOM["z_size"] / (GEOM["n_grid_z"] - 1))
 

@@ -7,7 +7,9 @@ FP64 plain form: every field frame to the goldens' 6 printed digits, and the
 reference's own acceptance (np.allclose, rtol 1e-3, atol 1, NaNs removed,
 test/functional/test.py:152-171) for all twelve probes including the two
 temperature frames.  FP32 form (through the in-process entry point): the same
-frames to 1e-3 normwise.
+frames to 1e-3 normwise.  The temperature and density probes run on the
+device (pdp3_temperature, pdp3_density); the density frame is compared with
+the oracle's ParticlesDensity on the same state.
 """
 import os
 import subprocess
@@ -24,8 +26,14 @@ TEMPLATE = os.path.join(HERE, "golden", "smoke", "parameters.xml.in")
 GOLDEN = os.path.join(HERE, "golden", "smoke")
 
 
+# a density probe (src/loadInitParam.cpp:381-390) beside the template's own probes
+DENSITY_PROBE = ('<probe type="frame" r_start="100" z_start="100" r_end="150" z_end="150" schedule="5" '
+                 'component="density" specie="Electrons" />')
+
+
 def _params(tmp_path, precision):
     text = open(TEMPLATE).read().replace("@RESULT_PATH@", str(tmp_path)).replace("@PRECISION@", precision)
+    text = text.replace("</probes>", DENSITY_PROBE + "\n    </probes>")
     p = tmp_path / "parameters.xml"
     p.write_text(text)
     return str(p)
@@ -92,6 +100,35 @@ def test_driver_fp64_temperature(fp64_run, specie):
     big = np.abs(ref) > 1e-6
     assert big.sum() > 10
     assert np.max(np.abs(ours[big] - ref[big]) / np.abs(ref[big])) < 1e-3
+
+
+def test_driver_fp64_density(fp64_run, orc):
+    """The density probe frame against ParticlesDensity::calc_density of the
+    oracle on the same loop state (the smoke replay to the fifth dump)."""
+    root, _ = fp64_run
+    ours = _read(_frame_path(root, "density", "Electrons")).reshape(50, 50)
+    plasma, bunch = sc.initial_state(orc)
+    g, T = sc.GEOM, sc.TIME
+    sim = orc.Sim(g["n_grid_r"], g["n_grid_z"], g["r_size"], g["z_size"], T["dt"],
+                  current_time=T["current_time"], end_time=T["end_time"])
+    for p in plasma:
+        sim.add_plasma(p["charge"], p["mass"], p["r"], p["z"], p["vr"], p["vphi"], p["vz"], p["w"])
+    b = bunch
+    sim.add_bunch(b["charge"], b["mass"], b["r"], b["w"], b["inject_u"], b["duration"], b["hole_duration"],
+                  b["velocity"], b["bunch_number"])
+    dumps, ref = 0, None
+    while ref is None:
+        fires = sc.probe_fires(sim.time, T["dt"])
+        sim.step(1)
+        if fires:
+            if dumps == sc.FRAME_NO:
+                ref = sc.t_frame(sim.density(0))
+            dumps += 1
+    assert np.count_nonzero(ref) > 100
+    # the file holds 6 significant digits
+    big = np.abs(ref) > 1e-6 * np.max(np.abs(ref))
+    assert np.max(np.abs(ours[big] - ref[big]) / np.abs(ref[big])) < 2e-5
+    assert np.max(np.abs(ours[~big])) <= 1e-6 * np.max(np.abs(ref))
 
 
 def test_driver_fp64_layout(fp64_run):

@@ -5,9 +5,19 @@ bit-identical to the oracle: both evaluate every expression in the
 reference's order without FMA contraction.  Grid sums are order-dependent
 (hardware FP64 atomics vs the reference's serial loop), so J / rho and
 anything downstream are compared normwise: <= 1e-13 after one deposit and,
-after 100 steps, <= 1e-10 or 10x the largest change the oracle itself shows
-when its serial deposit sums are reordered (reversed or shuffled particle
-order), whichever is larger.
+after 100 steps:
+
+* <= 1e-10, strictly, with the Vay pusher (the 64x256 plasma, and the full
+  64x256 configuration with its 0.9c bunch);
+* with the Boris pusher the reference's loop itself moves by more than 1e-10
+  when only the order of its serial deposit sums changes, or when only the
+  last bit of lib::get_gamma_inv changes in the few hundredths of a percent
+  of arguments where glibc's pow(y, -0.5) is not the double nearest to
+  1/sqrt(y) (the device computes the nearest): on the plasma alone H_phi
+  moves by 2.5e-10 and J by 4.4e-9 (a particle reflected at the axis one step
+  earlier or later), with the bunch E_z by ~7e-9.  There the bar is 10x the
+  largest such self-change of the oracle, and never more than a fixed ceiling
+  (2e-8 for fields, 5e-8 for particle state) set from those recorded floors.
 """
 import numpy as np
 import pytest
@@ -154,9 +164,11 @@ def test_charge_weighting_matches_oracle(capi, orc):
 def _run_pair(capi, orc, w, nsteps):
     """GPU and oracle from the same initial state, plus the oracle with its
     serial deposit sums taken in other orders (reversed, and two shuffled
-    particle orders): how far the reference's own loop moves when only the
-    order of the grid sums changes, which any parallel deposit does.  Returns
-    the GPU errors and that self-floor (max over the orders)."""
+    particle orders) and the oracle with get_gamma_inv taken as the double
+    nearest to 1/sqrt(y): how far the reference's own loop moves when only
+    the order of the grid sums changes, which any parallel deposit does, or
+    only the last-bit cases of glibc's pow.  Returns the GPU errors and that
+    self-floor (max over the variants)."""
     from pdp3_amd import workloads as wl
 
     gpu = wl.build_simulation(w)
@@ -171,12 +183,19 @@ def _run_pair(capi, orc, w, nsteps):
     floor = {k: 0.0 for k in errs}
     sp_floor = [dict(alive_mismatch=0, r=0.0, z=0.0, vr=0.0, vphi=0.0, vz=0.0) for _ in range(nsp)]
     rng = np.random.default_rng(99)
-    for variant in ("reverse", "shuffle", "shuffle"):
-        perms = [np.arange(len(st["r"])) if variant == "reverse" else rng.permutation(len(st["r"])) for st in states]
+    for variant in ("reverse", "shuffle", "shuffle", "nearest"):
+        if variant == "shuffle":
+            perms = [rng.permutation(len(st["r"])) for st in states]
+        else:
+            perms = [np.arange(len(st["r"])) for st in states]
         alt_states = [{k: v[p] for k, v in st.items()} for st, p in zip(states, perms)]
         o2 = oracle_from_workload(orc, w, gpu, alt_states)
         o2.set_reverse_deposit(variant == "reverse")
-        o2.step(nsteps)
+        orc.lib().or_set_gamma_inv_nearest(1 if variant == "nearest" else 0)
+        try:
+            o2.step(nsteps)
+        finally:
+            orc.lib().or_set_gamma_inv_nearest(0)
         for k, e in compare_fields(o2, ora).items():
             floor[k] = max(floor[k], e)
         for j in range(nsp):
@@ -189,43 +208,53 @@ def _run_pair(capi, orc, w, nsteps):
     return errs, floor, sp, sp_floor
 
 
-def _check(errs, floor, sp, sp_floor):
-    """FP64 bar: 1e-10 normwise, or 10x the oracle's own reordering floor
-    where the reference's loop is that sensitive."""
+FIELD_CEILING = 2e-8
+STATE_CEILING = 5e-8
+
+
+def _check(errs, floor, sp, sp_floor, strict):
+    """FP64 bar: 1e-10 normwise when strict; otherwise 1e-10, or 10x the
+    oracle's own self-change where the reference's loop is that sensitive,
+    capped by the fixed ceilings."""
     print("fields", errs)
-    print("oracle reorder floor", floor)
+    print("oracle self-change floor", floor)
     print("species", sp)
-    print("species reorder floor", sp_floor)
+    print("species self-change floor", sp_floor)
     for name, e in errs.items():
-        assert e <= max(1e-10, 10 * floor[name]), (name, e, floor[name])
+        bar = 1e-10 if strict else min(FIELD_CEILING, max(1e-10, 10 * floor[name]))
+        assert e <= bar, (name, e, floor[name])
     for cs, fl in zip(sp, sp_floor):
-        assert cs["alive_mismatch"] <= fl["alive_mismatch"]
+        assert cs["alive_mismatch"] <= (0 if strict else fl["alive_mismatch"])
         for k in ("r", "z", "vr", "vphi", "vz"):
-            assert cs[k] <= max(1e-10, 10 * fl[k]), (k, cs[k], fl[k])
+            bar = 1e-10 if strict else min(STATE_CEILING, max(1e-10, 10 * fl[k]))
+            assert cs[k] <= bar, (k, cs[k], fl[k])
 
 
 @pytest.mark.parametrize("pusher", [0, 1])
 def test_cfg1_plasma_100_steps(capi, orc, pusher):
     """The 64x256 configuration without the bunch (10 ppc plasma at 1e17,
-    1 eV, PML 8 cells), 100 full steps."""
+    1 eV, PML 8 cells), 100 full steps: Vay within 1e-10 strictly, Boris
+    within the oracle's own self-change (one axis reflection shifted by a
+    step moves J by ~4e-9)."""
     from pdp3_amd import workloads as wl
 
     w = wl.config("cfg1", pusher=pusher)
     w.bunches = []
-    _check(*_run_pair(capi, orc, w, 100))
+    _check(*_run_pair(capi, orc, w, 100), strict=(pusher == 1))
 
 
 @pytest.mark.parametrize("pusher", [0, 1])
 def test_cfg1_shape_100_steps(capi, orc, pusher):
     """The full 64x256 configuration (plasma + uniform 0.9c bunch, PML 8
-    cells) for 100 full steps.  The reference's J_r formula cancels
-    catastrophically for near-axial bunch motion, so its loop amplifies
-    round-off: reordering the oracle's own serial deposit sums already moves
-    E and J by ~1e-9 here."""
+    cells) for 100 full steps.  With Boris the reference's J_r formula
+    cancels catastrophically for near-axial bunch motion, so its loop
+    amplifies round-off: reordering the oracle's own serial deposit sums, or
+    changing only glibc's last-bit pow cases, already moves E and J by ~1e-9
+    here.  With Vay the 1e-10 bar holds strictly."""
     from pdp3_amd import workloads as wl
 
     w = wl.config("cfg1", pusher=pusher)
-    _check(*_run_pair(capi, orc, w, 100))
+    _check(*_run_pair(capi, orc, w, 100), strict=(pusher == 1))
 
 
 def test_nonfinite_position_is_reported(capi):

@@ -36,6 +36,9 @@ def smoke_frames(orc):
             if dumps == sc.FRAME_NO:
                 out = {c: sc.frame(sim.field("rho_bunch") if c == "rho_beam" else sim.field(c), c).copy()
                        for c in sc.FRAMES}
+                # TemperatureClassic probes of the two plasma species
+                for j, sp in enumerate(sc.SPECIES):
+                    out["T_" + sp["name"]] = sc.t_frame(sim.temperature(j))
                 out["_iteration"] = it
             dumps += 1
         it += 1
@@ -58,3 +61,19 @@ def test_frame_matches_golden(smoke_frames, comp):
     assert small_max < 1e-9, (comp, small_max)
     # the reference's own acceptance test
     assert np.allclose(ours, ref, rtol=1e-3, atol=1)
+
+
+@pytest.mark.parametrize("specie", [sp["name"] for sp in sc.SPECIES])
+def test_temperature_frame_matches_golden(smoke_frames, specie):
+    """TemperatureClassic (src/temperatureClassic.cpp:57-158) of the oracle
+    against the reference's T frames: the same empty cells (NaN from 0/0), the
+    reference's acceptance rule, and 2e-5 relative on the cells whose
+    temperature is not round-off (one-particle cells give S^2 - |V|^2 of a
+    few ulps)."""
+    ours, ref = smoke_frames["T_" + specie], sc.golden_t(specie)
+    assert ours.shape == ref.shape
+    assert np.array_equal(np.isnan(ours), np.isnan(ref))
+    assert np.allclose(ours, ref, rtol=1e-3, atol=1, equal_nan=True)
+    big = np.abs(np.nan_to_num(ref)) > 1e-6
+    assert big.sum() > 20
+    assert np.max(np.abs(ours[big] - ref[big]) / np.abs(ref[big])) < 2e-5
