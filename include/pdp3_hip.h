@@ -213,6 +213,20 @@ int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
  * doubles; R is 0 at k = 0, k = n_grid_z-1 and i = n_grid_r-1. */
 int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out);
 
+/* pdp3_gauss_residual without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  rho and
+ * the residual are computed in a fixed device array of the library
+ * (2 x PDP3_NOALLOC_MAX_NODES doubles per GPU, held until the process ends)
+ * by the same kernels; the outputs are as pdp3_gauss_residual's (host
+ * arrays, either may be NULL).  A grid of more than PDP3_NOALLOC_MAX_NODES
+ * nodes (n_grid_r * n_grid_z) is rejected with PDP3_ERR_BAD_ARG.  Every call
+ * on one GPU uses that GPU's one array, which keeps the last call's values:
+ * calls that overlap on one GPU (contexts driven from several host threads)
+ * race. */
+#define PDP3_NOALLOC_MAX_NODES 4718592 /* 4.5 Mi nodes, e.g. 513 x 8193 */
+int pdp3_gauss_residual_noalloc(pdp3_ctx *ctx, double *residual_out, double *rho_out);
+
 /* TemperatureClassic::calc_t + normalize of one species
  * (src/temperatureClassic.cpp:57-158, the "T" probe of LoadInitParam::dump_data,
  * src/loadInitParam.cpp:369-379), computed on the device from the resident

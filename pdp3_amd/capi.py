@@ -24,6 +24,7 @@ COMPONENTS = {
     "rho_bunch": 12, "sigma": 13, "epsilon": 14,
 }
 KERNEL_PARTICLES, KERNEL_H, KERNEL_E, KERNEL_SORT, KERNEL_BUNCH = range(5)
+NOALLOC_MAX_NODES = 4718592  # PDP3_NOALLOC_MAX_NODES
 TANGENT_DIRS = 4  # PDP3_TANGENT_DIRS
 
 _D = C.POINTER(C.c_double)
@@ -68,6 +69,7 @@ SIGNATURES = {
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
+    "pdp3_gauss_residual_noalloc": (_I, [_P, _D, _D]),
     "pdp3_temperature": (_I, [_P, _I, _D]),
     "pdp3_density": (_I, [_P, _I, _D]),
     "pdp3_bunch_add_loaded": (_I, [_P, _I, _F, _I, _D, _D, _D, _D, _D, _D, C.POINTER(_I)]),
@@ -282,6 +284,14 @@ class Simulation:
         R = np.empty((self.nr, self.nz))
         rho = np.empty((self.nr, self.nz))
         self._chk(self.L.pdp3_gauss_residual(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
+        return R, rho
+
+    def gauss_residual_noalloc(self):
+        """gauss_residual computed in the library's fixed device array (no
+        device allocation; grids of at most NOALLOC_MAX_NODES nodes)."""
+        R = np.empty((self.nr, self.nz))
+        rho = np.empty((self.nr, self.nz))
+        self._chk(self.L.pdp3_gauss_residual_noalloc(self.h, R.ctypes.data_as(_D), rho.ctypes.data_as(_D)))
         return R, rho
 
     def temperature(self, idx):

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <functional>
@@ -22,7 +23,33 @@ struct pdp3_ctx {
 
 using namespace pdp3;
 
+// Device memory of pdp3_gauss_residual_noalloc: the charge density of all
+// species (first PDP3_NOALLOC_MAX_NODES values), then the residual.  Every GPU
+// the library runs on holds its own copy (2 x 36 MiB) until the process ends.
+// In host code `scratch` is only a stand-in for the symbol: its device address
+// on a GPU comes from hipGetSymbolAddress (noalloc_address below).
+static __device__ __align__(256) double scratch[2 * (size_t)PDP3_NOALLOC_MAX_NODES];
+
 namespace {
+
+// per-device address of `scratch`, resolved once per device with that device
+// current (at the first context created on it)
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+bool g_noalloc_tried[kMaxDevices] = {};
+
+double *noalloc_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_tried[device]) {
+    g_noalloc_tried[device] = true;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_noalloc_ptr[device] = (double *)p;
+    else (void)hipGetLastError();  // leave no sticky error for the caller's next call
+  }
+  return g_noalloc_ptr[device];
+}
 
 int fail(pdp3_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->c.last_error = msg;
@@ -283,6 +310,7 @@ int pdp3_ctx_create(int n_grid_r, int n_grid_z, double r_size, double z_size, do
     return code;
   };
   if (hipGetDevice(&c.device) != hipSuccess) return bail(PDP3_ERR_HIP);
+  c.noalloc = noalloc_address(c.device);  // checked by pdp3_gauss_residual_noalloc
   if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(PDP3_ERR_HIP);
   const size_t n = c.cells();
   if ((st = dalloc(ctx, &c.grid, n * kNComp))) return bail(st);
@@ -872,6 +900,18 @@ int pdp3_gauss_residual(pdp3_ctx *ctx, double *residual_out, double *rho_out) {
   if (!c.rho_all && (st = dalloc(ctx, &c.rho_all, c.cells()))) return st;
   if (!c.gauss && (st = dalloc(ctx, &c.gauss, c.cells()))) return st;
   return gauss_into(ctx, c.rho_all, c.gauss, residual_out, rho_out);
+}
+
+int pdp3_gauss_residual_noalloc(pdp3_ctx *ctx, double *residual_out, double *rho_out) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  Ctx &c = ctx->c;
+  if (c.cells() > (size_t)PDP3_NOALLOC_MAX_NODES)
+    return fail(ctx, PDP3_ERR_BAD_ARG,
+                "grid has " + std::to_string(c.cells()) + " nodes; pdp3_gauss_residual_noalloc takes at most " +
+                    std::to_string((long long)PDP3_NOALLOC_MAX_NODES) + " (PDP3_NOALLOC_MAX_NODES)");
+  if (!c.noalloc)
+    return fail(ctx, PDP3_ERR_HIP, "the device address of the no-allocation array could not be resolved");
+  return gauss_into(ctx, c.noalloc, c.noalloc + (size_t)PDP3_NOALLOC_MAX_NODES, residual_out, rho_out);
 }
 
 int pdp3_temperature(pdp3_ctx *ctx, int index, double *out) {

@@ -159,6 +159,9 @@ struct Ctx {
   // probe workspace: 5 per-cell sums of the temperature probe, then the
   // probe output (6 x cells doubles, allocated by the first probe)
   double *probe_ws = nullptr;
+  // device address of the library's fixed no-allocation array on this
+  // context's GPU (nullptr when it could not be resolved)
+  double *noalloc = nullptr;
   // FP32 form: per radial cell index i (0 .. nr-1) two float4 entries,
   // {1/(2i), 1/(2(i+1)), 1/(2(i+2)), 1/(2(i+3))} and {1/i, 1/(i+1), 1/(i+1/2),
   // i(i+1)}, with 1/0 stored as 0 (built at context creation)
