@@ -297,6 +297,19 @@ int pdp3_adjoint_tangent_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, in
                              double z_hi, const double *const *dw, double *phi_out, double *const *dphi_dw,
                              double *dphi_tangent);
 
+/* pdp3_adjoint_run over a batch of independent members (no reference
+ * counterpart; e.g. the same problem at n parameter points): each member's
+ * context is driven by its own host thread on its own stream, so the
+ * members' forward sweeps, recomputations and adjoint kernels interleave on
+ * the device.  Member m writes phi_out[m] and dphi_dw[m][j] (species j, as
+ * pdp3_adjoint_run; dphi_dw or dphi_dw[m] may be NULL); status_out[m]
+ * (may be NULL) receives its status, the return value is the first nonzero
+ * one.  Every member holds its own checkpoints at once; a context may appear
+ * only once. */
+int pdp3_adjoint_run_batch(pdp3_ctx *const *ctxs, int n, int nsteps, int checkpoint_every, int window_steps,
+                           double z_lo, double z_hi, double *phi_out, double *const *const *dphi_dw,
+                           int *status_out);
+
 /* the same objective without the gradient (forward only) */
 int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi, double *phi_out);
 

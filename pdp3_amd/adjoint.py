@@ -198,6 +198,27 @@ class BumpRampProblem:
         return phi, dphi[:k]
 
 
+    def batch_gradients(self, thetas, precision=capi.PREC_FP32):
+        """phi and dphi/dtheta at several parameter points in one batched
+        adjoint run (pdp3_adjoint_run_batch: the members run concurrently on
+        their own streams): (phis [m], grads [m, len(theta)])"""
+        s = self.spec
+        thetas = [torch.as_tensor(t, dtype=torch.float64).detach() for t in thetas]
+        sims = [self.build(t, precision) for t in thetas]
+        try:
+            phis, grads_w = capi.adjoint_run_batch(sims, s.steps, s.checkpoint_every, s.window_steps, *s.window)
+        finally:
+            for sim in sims:
+                sim.close()
+        out = []
+        for t, gw in zip(thetas, grads_w):
+            th = t.clone().requires_grad_(True)
+            ws = self.weights(th)
+            (g,) = torch.autograd.grad(ws, th, grad_outputs=[torch.tensor(a, dtype=torch.float64) for a in gw])
+            out.append(g)
+        return np.array(phis), torch.stack(out).numpy()
+
+
 class PicObjective(torch.autograd.Function):
     """phi = PicObjective.apply(theta, problem[, precision]): the objective of
     the r-z PIC loop; backward through the library's adjoint."""

@@ -77,6 +77,8 @@ SIGNATURES = {
     "pdp3_objective_run": (_I, [_P, _I, _I, _F, _F, _D]),
     "pdp3_tangent_run": (_I, [_P, _I, _I, _F, _F, C.POINTER(_D), _D, _D]),
     "pdp3_adjoint_tangent_run": (_I, [_P, _I, _I, _I, _F, _F, C.POINTER(_D), _D, C.POINTER(_D), _D]),
+    "pdp3_adjoint_run_batch": (_I, [C.POINTER(_P), _I, _I, _I, _I, _F, _F, _D, C.POINTER(C.POINTER(_D)),
+                                    C.POINTER(_I)]),
     "pdp3_ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
     "pdp3_ensemble_destroy": (_I, [_P]),
     "pdp3_ensemble_last_error": (C.c_char_p, [_P]),
@@ -389,6 +391,28 @@ class Simulation:
         ms, n = C.c_double(), C.c_longlong()
         self._chk(self.L.pdp3_profile_get(self.h, kernel, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def adjoint_run_batch(sims, nsteps, checkpoint_every, window_steps, z_lo, z_hi):
+    """Simulation.adjoint_run over independent members at once (one host
+    thread and stream per member): ([phi per member], [[dphi/dw per species]
+    per member]); every simulation advances."""
+    L = lib()
+    n = len(sims)
+    handles = (_P * n)(*[s.h for s in sims])
+    counts = [[s.count(j) for j in range(s.n_species())] for s in sims]
+    outs = [[np.zeros(max(c, 1)) for c in cs] for cs in counts]
+    rows = [(_D * max(len(o), 1))(*[_dp(a) for a in o]) for o in outs]
+    table = (C.POINTER(_D) * n)(*[C.cast(r, C.POINTER(_D)) for r in rows])
+    phi = np.zeros(n)
+    status = (_I * n)()
+    st = L.pdp3_adjoint_run_batch(handles, n, int(nsteps), int(checkpoint_every), int(window_steps), float(z_lo),
+                                  float(z_hi), _dp(phi), table, status)
+    if st != OK:
+        bad = [m for m in range(n) if status[m] != OK]
+        msg = "; ".join(f"member {m}: {L.pdp3_last_error(sims[m].h).decode()}" for m in bad)
+        raise Pdp3Error(st, msg)
+    return list(phi), [[a[:c] for a, c in zip(o, cs)] for o, cs in zip(outs, counts)]
 
 
 class Ensemble:

@@ -13,6 +13,7 @@
 #include <new>
 #include <string>
 #include <functional>
+#include <thread>
 #include <vector>
 
 #include "context.hpp"
@@ -1313,6 +1314,38 @@ int pdp3_adjoint_tangent_run(pdp3_ctx *ctx, int nsteps, int checkpoint_every, in
                              double *dphi_tangent) {
   if (!dw) return PDP3_ERR_BAD_ARG;
   return adjoint_impl(ctx, nsteps, checkpoint_every, window_steps, z_lo, z_hi, phi_out, dphi_dw, dw, dphi_tangent);
+}
+
+int pdp3_adjoint_run_batch(pdp3_ctx *const *ctxs, int n, int nsteps, int checkpoint_every, int window_steps,
+                           double z_lo, double z_hi, double *phi_out, double *const *const *dphi_dw,
+                           int *status_out) {
+  if (!ctxs || n < 1) return PDP3_ERR_BAD_ARG;
+  for (int m = 0; m < n; m++) {
+    if (!ctxs[m]) return PDP3_ERR_BAD_ARG;
+    for (int q = 0; q < m; q++)
+      if (ctxs[q] == ctxs[m]) return fail(ctxs[m], PDP3_ERR_BAD_ARG, "a context appears twice in the batch");
+  }
+  // one host thread per member, each driving its own context (and stream);
+  // the members' launches interleave on the device
+  std::vector<int> st((size_t)n, PDP3_OK);
+  std::vector<std::thread> workers;
+  workers.reserve((size_t)n);
+  for (int m = 0; m < n; m++)
+    workers.emplace_back([&, m]() {
+      if (hipSetDevice(ctxs[m]->c.device) != hipSuccess) {
+        st[m] = fail(ctxs[m], PDP3_ERR_HIP, "hipSetDevice failed in the batch worker");
+        return;
+      }
+      st[m] = adjoint_impl(ctxs[m], nsteps, checkpoint_every, window_steps, z_lo, z_hi,
+                           phi_out ? phi_out + m : nullptr, dphi_dw ? dphi_dw[m] : nullptr, nullptr, nullptr);
+    });
+  for (auto &w : workers) w.join();
+  int first = PDP3_OK;
+  for (int m = 0; m < n; m++) {
+    if (status_out) status_out[m] = st[m];
+    if (first == PDP3_OK) first = st[m];
+  }
+  return first;
 }
 
 int pdp3_tangent_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo, double z_hi,

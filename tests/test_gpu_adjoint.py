@@ -153,3 +153,29 @@ def test_adjoint_fp32_matches_fd(problem_and_fd):
     phi.backward()
     print("phi fp32", float(phi), "oracle", phi0)
     _check(theta.grad.numpy(), fd, 2e-2, 2e-2)
+
+
+def test_batched_gradients_match_single_runs():
+    """pdp3_adjoint_run_batch (members concurrent on their own streams) gives
+    each member the gradient of its own single run (FP64 form: the runs agree
+    to round-off over the 200 steps of the reduced problem)."""
+    import torch
+
+    from pdp3_amd import adjoint as adj
+    from pdp3_amd import capi
+
+    prob = _problem()
+    th = prob.theta0()
+    rng = np.random.default_rng(3)
+    thetas = [th * torch.tensor(rng.uniform(0.8, 1.2, len(th))) for _ in range(3)]
+    print("batched gradient of 3 members vs single runs", flush=True)
+    phis, grads = prob.batch_gradients(thetas, capi.PREC_FP64)
+    for m, t in enumerate(thetas):
+        theta = t.clone().requires_grad_(True)
+        phi = adj.PicObjective.apply(theta, prob, capi.PREC_FP64)
+        phi.backward()
+        g = theta.grad.numpy()
+        e_phi = abs(phis[m] - float(phi)) / abs(float(phi))
+        e_g = np.max(np.abs(grads[m] - g)) / np.max(np.abs(g))
+        print("member", m, "phi rel diff", e_phi, "gradient normwise diff", e_g)
+        assert e_phi <= 1e-10 and e_g <= 1e-8, (m, e_phi, e_g)
