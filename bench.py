@@ -239,7 +239,7 @@ def bench_ensemble(a):
     n_part = sum(i.particles for i in insts)
     cells = sum(i.cells_r * i.cells_z for i in insts)
     print(f"# ensemble of {len(insts)} instances, {n_part} particles, {cells} cells", file=sys.stderr, flush=True)
-    sims = [wl.build_instance(i) for i in insts]
+    sims = [wl.build_instance(i, injected=not a.loaded_bunches) for i in insts]
     ens = capi.Ensemble(sims)
     ens.set_sort_interval(a.sort_interval)
     tile = ens.autotune(1)
@@ -255,7 +255,8 @@ def bench_ensemble(a):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * (t1 - t0) / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"cfg4: {len(insts)} r-z instances, r cells in 64/128/256 x z cells in "
-                               f"512..4096, 5..40 ppc, densities 1e16..1e18, Gaussian bunches",
+                               f"512..4096, 5..40 ppc, densities 1e16..1e18, Gaussian bunches "
+                               f"{'loaded at t = 0' if a.loaded_bunches else 'injected over 6 sigma_z / v'}",
                    "particles": n_part, "cells": cells, "tile_ppt_waves": list(tile),
                    "sort_interval": a.sort_interval},
         "device_ms_per_step": dev_ms / max(n, 1),
@@ -347,6 +348,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     ap.add_argument("--instances", type=int, default=128, help="cfg4: ensemble size")
+    ap.add_argument("--loaded-bunches", action="store_true",
+                    help="cfg4: bunches loaded inside the grid at t = 0 instead of injected over time")
     ap.add_argument("--fd", default="", help="cfg5: comma-separated parameter indices to check by differences")
     ap.add_argument("--members", type=int, default=0, help="cfg5: gradients of this many perturbed members")
     ap.add_argument("--adj-steps", type=int, default=0, help="cfg5: override the step count (default 1500)")

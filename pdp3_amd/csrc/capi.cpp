@@ -229,8 +229,9 @@ int inject_bunch(pdp3_ctx *ctx, Species &sp) {
   return PDP3_OK;
 }
 
-// LoadInitParam::init_beam, src/loadInitParam.cpp:168-248
-int init_beam(pdp3_ctx *ctx) {
+// LoadInitParam::init_beam, src/loadInitParam.cpp:168-248 (inject false: the
+// bookkeeping only, for ensembles, whose injections run on the device)
+int init_beam(pdp3_ctx *ctx, bool inject = true) {
   Ctx &c = ctx->c;
   int cand = -1;
   for (int j = 0; j < (int)c.species.size(); j++) {
@@ -246,6 +247,7 @@ int init_beam(pdp3_ctx *ctx) {
       c.next_bunch++;
     }
   }
+  if (!inject) return PDP3_OK;
   for (int j : c.order)
     if (c.species[j].kind == PDP3_SPECIES_BUNCH) {
       int st = inject_bunch(ctx, c.species[j]);
@@ -1019,6 +1021,10 @@ int sort_plasma(pdp3_ctx *ctx, bool exact) {
     if (st) return fail(ctx, st == -3 ? PDP3_ERR_OOM : PDP3_ERR_HIP, "sort failed");
   }
   CHECK_LAUNCH(ctx);
+  // an exact sort sizes its record space for one run per particle (~24 bytes
+  // per particle): give that back down to what the periodic sorts use
+  if (exact && sort_shrink_ws(c, sort_periodic_bytes(c)))
+    return fail(ctx, PDP3_ERR_OOM, "sort workspace");
   return PDP3_OK;
 }
 }  // namespace
@@ -1561,6 +1567,12 @@ struct pdp3_ensemble {
   double timed_ms = 0;
   long long timed_steps = 0;
   std::string last_error;
+  // injected bunches: one item per (instance, bunch), their start tables and
+  // the device step counter the graph's injection launch reads
+  EnsInjectItem *inj = nullptr;
+  int n_inj = 0, inj_max = 0;
+  int *inj_tables = nullptr;
+  long long *step_dev = nullptr;
 };
 
 namespace {
@@ -1584,17 +1596,19 @@ void ens_drop_graph(pdp3_ensemble *e) {
   e->graph_tile = -1;
 }
 
-// one loop iteration of every instance (src/loadInitParam.cpp:471-587 without
-// init_beam: ensemble bunches are loaded, not injected), captured once per
+// one loop iteration of every instance (src/loadInitParam.cpp:471-587; the
+// bunch injections of init_beam from the device tables), captured once per
 // tile shape and replayed
 int ens_graph(pdp3_ensemble *e) {
   if (e->exec && e->graph_tile == e->tile) return PDP3_OK;
   ens_drop_graph(e);
   ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  ens_launch_inject(e->inj, e->n_inj, e->inj_max, e->step_dev, e->stream);
   ens_launch_h(e->fields, e->stream);
   ens_launch_zero(e->fields, e->stream);
   ens_launch_particles(e->parts, e->tile, e->stream);
   ens_launch_e(e->fields, e->stream);
+  ens_launch_count(e->step_dev, e->stream);
   hipGraph_t g = nullptr;
   ENS_HIP(e, hipStreamEndCapture(e->stream, &g));
   e->graph = g;
@@ -1609,6 +1623,93 @@ int ens_check(pdp3_ensemble *e) {
     int st = check_device_errors(ctx);
     if (st) return efail(e, st, std::string("instance: ") + ctx->c.last_error);
   }
+  return PDP3_OK;
+}
+
+// the injection tables: per instance, LoadInitParam::init_beam's creation of
+// bunches and Bunch::bunch_inject's start index (src/loadInitParam.cpp:168-248,
+// src/bunch.cpp:87-124) replayed on the host from the instance's clock,
+// accumulated as the loop accumulates it, one entry per ensemble step until
+// every injected bunch has finished
+int ens_inject_build(pdp3_ensemble *e) {
+  std::vector<EnsInjectItem> items;
+  std::vector<std::vector<int>> tables;
+  for (Ctx *cp : e->cs) {
+    Ctx &c = *cp;
+    double t = c.current_time, t_last = -1e300;
+    std::vector<int> injected;  // species index of each injected bunch
+    for (int j = 0; j < (int)c.species.size(); j++) {
+      const Species &sp = c.species[j];
+      if (sp.kind != PDP3_SPECIES_BUNCH || !(sp.duration > 0) || !sp.inject_u) continue;
+      injected.push_back(j);
+      t_last = std::max(t_last, sp.duration * (sp.bunch_number + 1) + sp.hole_duration * sp.bunch_number);
+    }
+    if (injected.empty()) continue;
+    std::vector<char> created;
+    for (const Species &sp : c.species) created.push_back(sp.created ? 1 : 0);
+    int next = c.next_bunch;
+    std::vector<std::vector<int>> tab(injected.size());
+    for (long long step = 0; t < t_last + 2 * c.g.dt; step++) {
+      if (step > 100000000LL) return efail(e, PDP3_ERR_BAD_ARG, "bunch injection spans more than 1e8 steps");
+      int cand = -1;
+      for (int j = 0; j < (int)c.species.size(); j++) {
+        const Species &sp = c.species[j];
+        if (sp.kind == PDP3_SPECIES_BUNCH && !created[j] && sp.bunch_number == next) cand = j;
+      }
+      if (cand >= 0 && t >= next * (c.species[cand].duration + c.species[cand].hole_duration)) {
+        created[cand] = 1;
+        next++;
+      }
+      for (size_t b = 0; b < injected.size(); b++) {
+        const Species &sp = c.species[injected[b]];
+        int start = -1;
+        const double begin = sp.duration * sp.bunch_number + sp.hole_duration * sp.bunch_number;
+        const double end = begin + sp.duration;
+        const int steps_amount = (int)std::ceil(sp.duration / c.g.dt);
+        if (created[injected[b]] && steps_amount > 0 && t >= begin && t < end) {
+          const unsigned pis = (unsigned)sp.n / (unsigned)steps_amount;
+          start = (int)((t - begin) / c.g.dt * pis);
+        }
+        tab[b].push_back(start);
+      }
+      t = t + c.g.dt;
+    }
+    for (size_t b = 0; b < injected.size(); b++) {
+      Species &sp = c.species[injected[b]];
+      EnsInjectItem it{};
+      it.g = c.g;
+      it.cell = (uint32_t *)sp.f32.cell;
+      it.fb = sp.f32.fz;
+      it.ur = sp.f32.vr;
+      it.uphi = sp.f32.vphi;
+      it.uz = sp.f32.vz;
+      it.u = sp.inject_u;
+      it.n = sp.n;
+      it.count = (int)((unsigned)sp.n / (unsigned)std::max(1, (int)std::ceil(sp.duration / c.g.dt)));
+      it.table_len = (int)tab[b].size();
+      it.dl = sp.velocity * c.g.dt;
+      it.velocity = sp.velocity;
+      items.push_back(it);
+      tables.push_back(std::move(tab[b]));
+    }
+  }
+  ENS_HIP(e, hipMalloc((void **)&e->step_dev, sizeof(long long)));
+  ENS_HIP(e, hipMemset(e->step_dev, 0, sizeof(long long)));
+  e->n_inj = (int)items.size();
+  if (items.empty()) return PDP3_OK;
+  size_t total = 0;
+  for (auto &t : tables) total += t.size();
+  ENS_HIP(e, hipMalloc((void **)&e->inj_tables, std::max<size_t>(total, 1) * sizeof(int)));
+  size_t off = 0;
+  for (size_t b = 0; b < items.size(); b++) {
+    ENS_HIP(e, hipMemcpy(e->inj_tables + off, tables[b].data(), tables[b].size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+    items[b].start = e->inj_tables + off;
+    off += tables[b].size();
+    e->inj_max = std::max(e->inj_max, items[b].count);
+  }
+  ENS_HIP(e, hipMalloc((void **)&e->inj, items.size() * sizeof(EnsInjectItem)));
+  ENS_HIP(e, hipMemcpy(e->inj, items.data(), items.size() * sizeof(EnsInjectItem), hipMemcpyHostToDevice));
   return PDP3_OK;
 }
 
@@ -1627,9 +1728,7 @@ int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out) {
       return PDP3_ERR_BAD_ARG;
     }
     Ctx &c = ctx->c;
-    bool ok = is_f32(c) && c.g.dt == ctxs[0]->c.g.dt;
-    for (const Species &sp : c.species)
-      if (sp.kind == PDP3_SPECIES_BUNCH && !(sp.created && sp.duration == 0)) ok = false;  // loaded bunches only
+    const bool ok = is_f32(c) && c.g.dt == ctxs[0]->c.g.dt;
     if (!ok) {
       delete e;
       return PDP3_ERR_BAD_ARG;
@@ -1652,6 +1751,10 @@ int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out) {
     pdp3_ensemble_destroy(e);
     return PDP3_ERR_OOM;
   }
+  if (int st = ens_inject_build(e)) {
+    pdp3_ensemble_destroy(e);
+    return st;
+  }
   *out = e;
   return PDP3_OK;
 }
@@ -1662,6 +1765,9 @@ int pdp3_ensemble_destroy(pdp3_ensemble *e) {
   ens_drop_graph(e);
   ens_fields_free(e->fields);
   ens_particles_free(e->parts);
+  if (e->inj) (void)hipFree(e->inj);
+  if (e->inj_tables) (void)hipFree(e->inj_tables);
+  if (e->step_dev) (void)hipFree(e->step_dev);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1695,6 +1801,10 @@ int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps) {
       if (ens_particles_update(e->parts, e->cs) != 0) return efail(e, PDP3_ERR_HIP, "ensemble tables");
     }
     if ((st = ens_graph(e))) return st;
+    // init_beam's bookkeeping (the bunches join the species list on the
+    // host; their injection runs in the graph)
+    for (pdp3_ctx *ctx : e->ctxs)
+      if ((st = init_beam(ctx, false))) return efail(e, st, ctx->c.last_error);
     if (!timing) {
       ENS_HIP(e, hipEventRecord(e->ev0, e->stream));
       timing = true;

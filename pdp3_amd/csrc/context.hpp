@@ -192,6 +192,23 @@ EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs);
 int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs);  // 0 on success
 void ens_particles_free(EnsParticles *e);
 void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st);
+// bunch injection of an ensemble (Bunch::bunch_inject, src/bunch.cpp:87-124)
+// driven by a device step counter: per (instance, bunch) pair a table of the
+// first particle injected at each ensemble step (-1: none), computed on the
+// host from the instance's accumulated clock; the counter is incremented by
+// the last launch of each step
+struct EnsInjectItem {
+  GridDesc g;
+  uint32_t *cell;
+  float *fb, *ur, *uphi, *uz;
+  const double *u;
+  const int *start;
+  int n, count, table_len;
+  double dl, velocity;
+};
+void ens_launch_inject(const EnsInjectItem *items, int n_items, int max_count, const long long *step,
+                       hipStream_t st);
+void ens_launch_count(long long *step, hipStream_t st);
 
 // adjoint of one loop iteration (FP64); lam = adjoints of E_r, E_phi, E_z,
 // H_r, H_phi, H_z, Hh_r, Hh_phi, Hh_z, J_r, J_phi, J_z (12 grids)
@@ -261,5 +278,10 @@ int sort_species(Ctx &c, Species &sp, bool exact);
 // run count), so that a loop that sorts allocates nothing after its first
 // step; returns 0, or -3 when out of memory
 int sort_prepare(Ctx &c, Species &sp);
+// workspace bytes the periodic sorts of the context's plasma species need
+// (their current record space), and a shrink of the shared workspace back to
+// that size after an exact sort grew it (waits for the stream; 0 or -3)
+size_t sort_periodic_bytes(const Ctx &c);
+int sort_shrink_ws(Ctx &c, size_t need);
 
 }  // namespace pdp3

@@ -934,12 +934,8 @@ __global__ void k_unpack(GridDesc g, double *r, double *z, double *vr, double *v
 
 // Bunch::bunch_inject (src/bunch.cpp:104-121) on the cell form: z from the
 // draw, v_z set to the bunch speed, the radial state kept
-__global__ void k_inject32(GridDesc g, uint32_t *cell, float *fb, float *ur, float *uphi, float *uz,
-                           const double *u, int n, unsigned start, unsigned count, double dl, double velocity) {
-  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  unsigned j = t + start;
-  if (j >= (unsigned)n) return;
+__device__ __forceinline__ void inject_one32(const GridDesc &g, uint32_t *cell, float *fb, float *ur, float *uphi,
+                                             float *uz, const double *u, unsigned j, double dl, double velocity) {
   double zabs = dl * u[j] + g.dz / 2.0;
   double xz = zabs / g.dz;
   int k = (int)ceil(xz) - 1;
@@ -954,6 +950,33 @@ __global__ void k_inject32(GridDesc g, uint32_t *cell, float *fb, float *ur, flo
   ur[j] = (float)(gam * v0);
   uphi[j] = (float)(gam * v1);
   uz[j] = (float)(gam * v2);
+}
+__global__ void k_inject32(GridDesc g, uint32_t *cell, float *fb, float *ur, float *uphi, float *uz,
+                           const double *u, int n, unsigned start, unsigned count, double dl, double velocity) {
+  unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  unsigned j = t + start;
+  if (j >= (unsigned)n) return;
+  inject_one32(g, cell, fb, ur, uphi, uz, u, j, dl, velocity);
+}
+
+// ensembles: the injections of every (instance, bunch) pair for the
+// ensemble's current step, read from the step counter on the device (so the
+// launch can sit in the step's graph); item y of the grid, particle x
+__global__ void k_ens_inject(const EnsInjectItem *__restrict__ items, const long long *__restrict__ step) {
+  const EnsInjectItem &it = items[blockIdx.y];
+  const long long s = *step;
+  if (s >= it.table_len) return;
+  const int start = it.start[s];
+  if (start < 0) return;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < (unsigned)it.count; t += gridDim.x * blockDim.x) {
+    const unsigned j = t + (unsigned)start;
+    if (j >= (unsigned)it.n) break;
+    inject_one32(it.g, it.cell, it.fb, it.ur, it.uphi, it.uz, it.u, j, it.dl, it.velocity);
+  }
+}
+__global__ void k_ens_count(long long *step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
 }
 
 F32Args args32(Ctx &c, Species &sp) {
@@ -1131,7 +1154,9 @@ void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::
   int a = 0;
   for (Ctx *cp : ctxs) {
     Ctx &c = *cp;
-    for (int j : c.order) {
+    // every species by index, a bunch whether or not it has been created
+    // (until its injection its particles are dead and do nothing)
+    for (int j = 0; j < (int)c.species.size(); j++) {
       Species &sp = c.species[j];
       if (sp.n_cap <= 0 || !sp.f32.cell) continue;
       args->push_back(args32(c, sp));
@@ -1151,7 +1176,7 @@ int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs) {
   std::vector<F32Args> args;
   std::vector<GlobalSink> sinks;
   for (Ctx *cp : ctxs)
-    for (int j : cp->order) {
+    for (int j = 0; j < (int)cp->species.size(); j++) {
       Species &sp = cp->species[j];
       if (sp.n_cap <= 0 || !sp.f32.cell) continue;
       args.push_back(args32(*cp, sp));
@@ -1201,6 +1226,17 @@ void ens_launch_t(EnsParticles *e, int t, hipStream_t st) {
   if (blocks <= 0) return;
   hipLaunchKernelGGL((k_ens_fused32<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, e->args, e->sinks,
                      e->items[t], e->n_items[t]);
+}
+
+void ens_launch_inject(const EnsInjectItem *items, int n_items, int max_count, const long long *step,
+                       hipStream_t st) {
+  if (n_items <= 0 || max_count <= 0) return;
+  const unsigned bx = (unsigned)std::min<long long>(((long long)max_count + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_ens_inject, dim3(bx, (unsigned)n_items), dim3(256), 0, st, items, step);
+}
+
+void ens_launch_count(long long *step, hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_count, dim3(1), dim3(64), 0, st, step);
 }
 
 void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st) {

@@ -510,6 +510,25 @@ bool sortable(const Ctx &c, const Species &sp) {
 
 }  // namespace
 
+size_t sort_periodic_bytes(const Ctx &c) {
+  size_t need = 0;
+  for (const Species &sp : c.species) {
+    if (sp.kind != 0 || !sortable(c, sp)) continue;
+    const long long cap = sp.rec_cap > 0 ? sp.rec_cap : std::min(sp.n_cap, sp.n_cap / 4 + 65536);
+    need = std::max(need, sort_ws(sp.n_cap, cap).bytes);
+  }
+  return need;
+}
+
+int sort_shrink_ws(Ctx &c, size_t need) {
+  if (c.sort_ws_bytes <= need + need / 2) return 0;
+  if (hipStreamSynchronize(c.stream) != hipSuccess) return -3;
+  (void)hipFree(c.sort_ws);
+  c.sort_ws = nullptr;
+  c.sort_ws_bytes = 0;
+  return need == 0 || ensure_ws(c, need) ? 0 : -3;
+}
+
 int sort_prepare(Ctx &c, Species &sp) {
   if (sp.sort_prepared || !sortable(c, sp)) return 0;
   const long long n = sp.n_cap;

@@ -228,8 +228,11 @@ def loaded_gaussian_bunch(rng, inst: Instance):
     return b.r, z, zero, zero, np.full(n, inst.bunch_velocity), b.w
 
 
-def build_instance(inst: Instance, precision: int = 1):
-    """A Simulation of one ensemble member (FP32 form by default)."""
+def build_instance(inst: Instance, precision: int = 1, injected: bool = False):
+    """A Simulation of one ensemble member (FP32 form by default).  The
+    Gaussian bunch is loaded inside the grid at t = 0, or, with injected,
+    enters through z = dz/2 over 6 sigma_z / v as the reference's Bunch does
+    (Bunch::bunch_inject, src/bunch.cpp:87-124)."""
     from .capi import Simulation
 
     w = Workload("cfg4", inst.cells_r, inst.cells_z, inst.dr, inst.dr, 1000, inst.pml_cells, 0, seed=inst.seed)
@@ -241,6 +244,12 @@ def build_instance(inst: Instance, precision: int = 1):
     idx = sim.add_species(-1, 1.0, inst.ppc * inst.cells_r * inst.cells_z)
     sim.init_uniform(idx, inst.seed, inst.ppc, inst.density, 0, inst.temperature_ev)
     rng = np.random.default_rng(inst.seed + 1)
-    sim.add_bunch_loaded(-1, 1.0, *loaded_gaussian_bunch(rng, inst))
+    if injected:
+        b = gaussian_bunch(rng, inst.bunch_macro, inst.sigma_r, inst.sigma_z, inst.bunch_ratio * inst.density,
+                           inst.bunch_velocity, w.dr, w.dz)
+        sim.add_bunch(b.charge, b.mass, b.r, b.w, b.inject_u, b.duration, b.hole_duration, b.velocity,
+                      b.bunch_number)
+    else:
+        sim.add_bunch_loaded(-1, 1.0, *loaded_gaussian_bunch(rng, inst))
     return sim
 

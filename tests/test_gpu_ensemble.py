@@ -1,7 +1,8 @@
 """Ensembles: independent instances of different shapes advanced together
 (graph replay, one persistent particle launch) give the same result as each
 instance stepped alone, with and without periodic cell sorts, for every tile
-shape the autotuner may pick."""
+shape the autotuner may pick, with bunches loaded at the start or injected
+over time (Bunch::bunch_inject from the device step counter)."""
 import numpy as np
 import pytest
 
@@ -28,7 +29,7 @@ def _compare(a, b, tol):
         # runs: compare the states in the order of (r, z)
         oa, ob = np.lexsort((sa["z"], sa["r"])), np.lexsort((sb["z"], sb["r"]))
         assert np.array_equal(sa["alive"][oa], sb["alive"][ob])
-        for k in ("r", "z", "vr", "vz"):
+        for k in ("r", "z", "vr", "vphi", "vz"):
             assert normwise(sa[k][oa], sb[k][ob]) <= tol, (j, k)
 
 
@@ -83,7 +84,7 @@ def test_ensemble_tiles_and_autotune():
     ens.close()
 
 
-def test_ensemble_rejects_fp64_and_injected_bunches():
+def test_ensemble_rejects_fp64():
     from pdp3_amd import capi
     from pdp3_amd import workloads as wl
 
@@ -91,7 +92,48 @@ def test_ensemble_rejects_fp64_and_injected_bunches():
     s64 = wl.build_instance(inst, precision=capi.PREC_FP64)
     with pytest.raises(capi.Pdp3Error):
         capi.Ensemble([s64])
-    w = wl.config("cfg1")
-    s = wl.build_simulation(w, precision=capi.PREC_FP32)   # its bunch is injected over time
-    with pytest.raises(capi.Pdp3Error):
-        capi.Ensemble([s])
+
+
+@pytest.mark.parametrize("sort_interval", [0, 10])
+def test_ensemble_injected_bunches(sort_interval):
+    """Instances whose bunches are injected over time (the 64 x 256
+    configuration's uniform 0.9c bunch, on two grids, one with a two-bunch
+    train) beside a loaded-bunch instance: the ensemble's device-side injection
+    follows each instance's own clock exactly as its single stepping does."""
+    from pdp3_amd import capi
+    from pdp3_amd import workloads as wl
+
+    def injected(cells_r, cells_z, seed, train):
+        w = wl.config("cfg1", seed=seed, cells_r=cells_r, cells_z=cells_z, pml_cells=4, bunch_macro=3000)
+        if train:
+            b0 = w.bunches[0]
+            b0.hole_duration = 0.4 * b0.duration
+            b1 = wl.Bunch(b0.charge, b0.mass, b0.r.copy(), b0.w.copy(), b0.inject_u[::-1].copy(), b0.duration,
+                          b0.hole_duration, b0.velocity, 1)
+            w.bunches.append(b1)
+        return w
+
+    ws = [injected(16, 96, 41, False), injected(24, 64, 42, True)]
+    print("ensemble with injected bunches, sort_interval", sort_interval, flush=True)
+    alone = [wl.build_simulation(w, precision=capi.PREC_FP32) for w in ws]
+    members = [wl.build_simulation(w, precision=capi.PREC_FP32) for w in ws]
+    inst = _instances()[0]
+    alone.append(wl.build_instance(inst))
+    members.append(wl.build_instance(inst))
+    for s in alone:
+        s.set_sort_interval(sort_interval)
+        s.step(60)
+    ens = capi.Ensemble(members)
+    ens.set_sort_interval(sort_interval)
+    ens.step(60)
+    ens.close()
+    for a, m in zip(alone, members):
+        assert abs(a.time - m.time) == 0.0
+        assert np.max(np.abs(m.field("rho_bunch"))) > 0
+        _compare(m, a, 1e-10)
+        for j in range(1, a.n_species()):  # the bunches (plain layout: same order in both)
+            sa, sm = a.download(j), m.download(j)
+            assert np.array_equal(sa["alive"], sm["alive"]), j
+            assert sa["alive"].sum() > 0
+            for k in ("r", "z", "vr", "vphi", "vz"):
+                assert normwise(sm[k], sa[k]) <= 1e-10, (j, k)

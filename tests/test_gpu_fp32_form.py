@@ -6,9 +6,13 @@ cannot be bit-compatible with the reference; its contract is accuracy:
   * packing an FP64 state and unpacking it again loses only FP32 rounding of
     the in-cell offset and of u;
   * one step from identical state agrees with FP64 to FP32 round-off;
-  * after 100 steps of the 64x256 configuration, E_z, E_r and H_phi have a
-    relative RMS error below 1e-3 against the oracle (the per-config error
-    statistics, including cfg3 at 1000 steps, are measured by bench.py).
+  * after 100 steps of the 64x256 configuration, E_z has a relative RMS error
+    of at most 1e-4 against the oracle (the precision target), E_r and H_phi
+    at most 1e-3, and J_z, J_r (which carry the particles' individual rounding
+    rather than a field's average) at most 1e-2.
+The reduced copies of cfg2, cfg3 and cfg4 are compared with the oracle in
+test_gpu_configs.py; bench.py reports cfg3's E_z error after its 1000 steps
+on the full grid against the GPU's FP64 plain form.
 """
 import numpy as np
 import pytest
@@ -93,8 +97,11 @@ def test_cfg1_100_steps_error(pusher, orc):
     g32.step(100)
     errs = {name: rel_rms(g32.field(name), ora.field(name)) for name in ("E_r", "E_z", "H_phi", "J_z", "J_r")}
     print("relative RMS errors after 100 steps", errs)
-    for name in ("E_r", "E_z", "H_phi"):
+    assert errs["E_z"] <= 1e-4, errs
+    for name in ("E_r", "H_phi"):
         assert errs[name] <= 1e-3, (name, errs[name])
+    for name in ("J_z", "J_r"):
+        assert errs[name] <= 1e-2, (name, errs[name])
 
 
 def test_tile_shapes_agree_and_autotune():
