@@ -128,6 +128,15 @@ __device__ __forceinline__ float gather(const F32Args &p, const double *__restri
 // cells can touch, and falls back to HBM for the rest.  Both evaluate the same
 // FP32 expression on the same FP32-rounded node values, so the result does not
 // depend on the source.
+// the two table entries of row i (the host builds p.rows with the same FP64
+// expressions, so both give the same FP32 values)
+__device__ __forceinline__ void row_entries(int i, float4 &t0, float4 &t1) {
+  const double fi = i;
+  t0 = make_float4(i ? (float)(1.0 / (2.0 * fi)) : 0.f, (float)(1.0 / (2.0 * (fi + 1))), (float)(1.0 / (2.0 * (fi + 2))),
+                   (float)(1.0 / (2.0 * (fi + 3))));
+  t1 = make_float4(i ? (float)(1.0 / fi) : 0.f, (float)(1.0 / (fi + 1)), (float)(1.0 / (fi + 0.5)), (float)(fi * (fi + 1)));
+}
+
 struct GlobalFields {
   // the two table entries of row i (clamped only for memory safety: a corrupt
   // cell word is caught by the gathers' domain check)
@@ -168,15 +177,17 @@ __device__ __forceinline__ float gather_box(const float2 *e, RadW rw, AxW aw) {
 struct BoxFields {
   const float *box;  // LDS, kFComp x kFR x kFK pairs, then the row entries of rows ai-1 .. ai+2
   int ai, ak;        // first cell of the block
+  // the box's row entries (an LDS read for every particle), replaced by the
+  // entries evaluated in place for a row outside the box.  A global load here
+  // would make the read a flat load, whose wait also waits for the next
+  // particle's state loads already in flight.
   __device__ __forceinline__ void row(const F32Args &p, int i, float4 &t0, float4 &t1) const {
     const int li = i - ai + 1;
-    if ((unsigned)li < (unsigned)kFR) {
-      const float4 *t = reinterpret_cast<const float4 *>(box + 2 * kFSize);
-      t0 = t[2 * li];
-      t1 = t[2 * li + 1];
-    } else {
-      GlobalFields().row(p, i, t0, t1);
-    }
+    const int lc = min(max(li, 0), kFR - 1);
+    const float4 *t = reinterpret_cast<const float4 *>(box + 2 * kFSize);
+    t0 = t[2 * lc];
+    t1 = t[2 * lc + 1];
+    if ((unsigned)li >= (unsigned)kFR) row_entries(min(max(i, 0), p.nr - 1), t0, t1);
   }
   __device__ __forceinline__ void get(const F32Args &p, int i, int k, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
                                       float h[3]) const {
@@ -405,6 +416,16 @@ __device__ void segment(const F32Args &p, S &sk, float w, int i, int k, float a,
   sk.template add2<kCR>(i, k, (double)(sr * jr0), (double)(sr * jr1));
 }
 
+// the path parameter in [0, 1] at which a move of `d` cells reaches a face
+// `f` cells away.  A move across a face can round to d = 0 in cell units: a
+// particle 4e-9 cells above a face that ends 1e-9 cells below it has the new
+// offset 1 - 1e-9, which is 1.0f, in the cell below, and the displacement
+// (-1) + (1.0f - 4e-9) is 0.  The face is then taken at the start of the path
+// (the deposited current is that of the part of the move that FP32 keeps).
+__device__ __forceinline__ float cross_param(float f, float d) {
+  return d != 0.f ? fminf(fmaxf(fdiv(f, d), 0.f), 1.f) : 0.f;
+}
+
 // current_distribution (src/particles.cpp:476-686): the straight path from
 // the old to the new position, split at the cell faces it crosses (at most
 // one radial and one axial face: |displacement| <= c dt <= dr/2), each piece
@@ -426,8 +447,8 @@ __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, in
   float ca = ao, cb = bo;  // current start point, offsets in cell (ci, ck)
   if (di != 0 || dk != 0) {
     float tr = 2.f, tz = 2.f;
-    if (di != 0) tr = fdiv((di > 0 ? 1.f : 0.f) - ao, drho);
-    if (dk != 0) tz = fdiv((dk > 0 ? 1.f : 0.f) - bo, dzeta);
+    if (di != 0) tr = cross_param((di > 0 ? 1.f : 0.f) - ao, drho);
+    if (dk != 0) tz = cross_param((dk > 0 ? 1.f : 0.f) - bo, dzeta);
     float tprev = 0.f;
     for (int pass = 0; pass < 2; pass++) {
       bool rfirst = tr <= tz;

@@ -82,6 +82,73 @@ def test_one_step_against_fp64(pusher, orc):
         assert e <= 1e-4, (k, e)
 
 
+@pytest.mark.parametrize("axis", ["z", "r"])
+def test_face_crossing_below_fp32_resolution(axis):
+    """Particles 4.2e-9 cells above a cell face move 5e-9 cells across it
+    (and 0.01 cells along the other axis): the new offset, 1 - 8e-10 in the
+    cell below, is 1.0f, so the displacement in cell units rounds to 0 while
+    the cell index changes.  The deposit must stay finite (it once divided by
+    that zero) and carry the current of the move that FP32 keeps (the 4.2e-9
+    cells up to the face); the run stays finite.  Against the FP64 form:
+      * the 0.01-cell component (J_r for the z crossing, J_z for the r
+        crossing) to 1e-4 normwise;
+      * the crossing component for the z crossing to 0.3 (FP64 puts 16% of it,
+        the 0.8e-9 cells past the face, in the cell below);
+      * the crossing component for the r crossing only in size (at most 1e-3
+        of the other component): the reference's J_r formula takes it as the
+        difference of O(r dz) terms and log((r + dr) / r) of a ratio within
+        1e-10 of 1, so the FP64 form's value is round-off at this step
+        (src/particles.cpp simple_current_distribution)."""
+    from pdp3_amd import capi
+
+    rng = np.random.default_rng(31)
+    nr, nz, dr = 33, 129, 5e-4
+    dt = 0.5 * dr / 3e8
+    n = 4096
+    i = rng.integers(3, 29, n)
+    k = rng.integers(3, 125, n)
+    r = (i + rng.uniform(0.1, 0.9, n)) * dr
+    z = (k + rng.uniform(0.1, 0.9, n)) * dr
+    v = np.zeros((3, n))
+    speed = 5e-9 * dr / dt  # 5e-9 cells per step
+    if axis == "z":
+        z = (k + 4.2e-9) * dr
+        v[2] = -speed
+        v[0] = 2e6 * speed  # 0.01 cells per step
+        cross, other = "J_z", "J_r"
+    else:
+        r = (i + 4.2e-9) * dr
+        v[0] = -speed
+        v[2] = 2e6 * speed
+        cross, other = "J_r", "J_z"
+    w = np.full(n, 1e6)
+    sims = [capi.Simulation(nr, nz, 32 * dr, 128 * dr, dt, precision=p) for p in (capi.PREC_FP64, capi.PREC_FP32)]
+    for s in sims:
+        s.add_plasma(-1, 1.0, r, z, v[0], v[1], v[2], w)
+        s.step(1)
+    for name in ("J_r", "J_phi", "J_z"):
+        assert np.all(np.isfinite(sims[1].field(name))), name
+    e_other = normwise(sims[1].field(other), sims[0].field(other))
+    print(axis, other, "FP32 vs FP64", e_other)
+    assert e_other <= 1e-4, e_other
+    if axis == "z":
+        e = normwise(sims[1].field(cross), sims[0].field(cross))
+        print(axis, cross, "FP32 vs FP64", e)
+        assert e <= 0.3, e
+    else:
+        ratio = np.max(np.abs(sims[1].field(cross))) / np.max(np.abs(sims[1].field(other)))
+        print(axis, cross, "max relative to", other, ratio)
+        assert ratio <= 1e-3, ratio
+    sims[1].step(3)
+    for name in ("E_r", "E_phi", "E_z", "H_r", "H_phi", "H_z", "J_r", "J_phi", "J_z"):
+        assert np.all(np.isfinite(sims[1].field(name))), name
+    out = sims[1].download(0)
+    for key in ("r", "z", "vr", "vphi", "vz"):
+        assert np.all(np.isfinite(out[key])), key
+    for s in sims:
+        s.close()
+
+
 @pytest.mark.parametrize("pusher", [0, 1])
 def test_cfg1_100_steps_error(pusher, orc):
     from pdp3_amd import capi
