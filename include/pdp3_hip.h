@@ -201,6 +201,16 @@ int pdp3_sync(pdp3_ctx *ctx);
 int pdp3_sort_particles(pdp3_ctx *ctx);
 int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval);
 
+/* No counterpart in the reference (an analysis option of the FP64 plain
+ * form): on != 0 evaluates the second piece of a strictly radial move that
+ * crosses a radial face (strict_motion_weighting, src/particles.cpp:937-945
+ * and :972-980) with r0*r0/(i_n*dr), the expression of its other pieces,
+ * where the reference's `r0 * r0 / i_n * dr` parses as ((r0*r0)/i_n)*dr and
+ * leaves a spurious J_r of order q r0 / (dt dz^2) at such an event.  Default
+ * 0 (the reference's arithmetic).  The FP32 form's closed forms have no such
+ * term either way. */
+int pdp3_set_strict_radial_fix(pdp3_ctx *ctx, int on);
+
 /* Gauss-law residual R = div E - rho/eps0 on the grid nodes (no reference
  * counterpart: the reference allocates c_rho_new for it but never fills it,
  * src/loadInitParam.cpp:147,534).  rho is the charge density of every living

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pdp3_sync": (_I, [_P]),
     "pdp3_sort_particles": (_I, [_P]),
     "pdp3_set_sort_interval": (_I, [_P, _I]),
+    "pdp3_set_strict_radial_fix": (_I, [_P, _I]),
     "pdp3_gauss_residual": (_I, [_P, _D, _D]),
     "pdp3_gauss_residual_noalloc": (_I, [_P, _D, _D]),
     "pdp3_temperature": (_I, [_P, _I, _D]),
@@ -379,6 +380,11 @@ class Simulation:
 
     def set_sort_interval(self, n):
         self._chk(self.L.pdp3_set_sort_interval(self.h, int(n)))
+
+    def set_strict_radial_fix(self, on=True):
+        """FP64 plain form: evaluate strictly radial face crossings without
+        the reference's operator-precedence slip (analysis option)"""
+        self._chk(self.L.pdp3_set_strict_radial_fix(self.h, 1 if on else 0))
 
     # ---- timing ----
     def profile(self, enable=True):

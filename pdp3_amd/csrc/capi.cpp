@@ -1040,6 +1040,12 @@ int pdp3_set_sort_interval(pdp3_ctx *ctx, int sort_interval) {
   return PDP3_OK;
 }
 
+int pdp3_set_strict_radial_fix(pdp3_ctx *ctx, int on) {
+  if (!ctx) return PDP3_ERR_BAD_ARG;
+  ctx->c.g.strict_fix = on ? 1 : 0;
+  return PDP3_OK;
+}
+
 int pdp3_set_tile(pdp3_ctx *ctx, int ppt, int waves) {
   if (!ctx) return PDP3_ERR_BAD_ARG;
   for (int t = 0; t < kNumTiles; t++)

@@ -42,6 +42,11 @@ struct GridDesc {
   double inv_dr, inv_dz;
   double cyl, inv_cyl;
   const double2 *vol;
+  // 1: the FP64 form evaluates the second piece of a strictly radial move
+  // across a radial face with r0^2 / (i dr), as its other pieces do, instead
+  // of the reference's r0 * r0 / i_n * dr (= (r0^2 / i) dr,
+  // src/particles.cpp:941, :976); 0 (default): as the reference
+  int strict_fix = 0;
 };
 
 // Physical layout of plasma species.  Particles are processed by wavefronts

@@ -50,7 +50,10 @@ struct F32Args {
   int is_bunch;
   int want_rho;
   int n;
-  const double *er, *ephi, *ez, *hhr, *hhphi, *hhz;
+  // the context's grid: component c (kER, kEPhi, kEZ, kHHR, kHHPhi, kHHZ) at
+  // grid + c * nr * nz (one base pointer instead of six keeps the kernels'
+  // scalar registers from spilling)
+  const double *grid;
   // per radial cell index i, two entries: {1/(2i), 1/(2(i+1)), 1/(2(i+2)),
   // 1/(2(i+3))} and {1/i, 1/(i+1), 1/(i+1/2), i(i+1)} (1/0 read as 0)
   const float4 *rows;
@@ -59,6 +62,11 @@ struct F32Args {
   float *fa, *fb, *ur, *uphi, *uz;
   const float *w;
 };
+
+// component c of the grid
+__device__ __forceinline__ const double *field(const F32Args &p, int c) {
+  return p.grid + (size_t)c * ((size_t)p.nr * (size_t)p.nz);
+}
 
 // radial weights of the four-node gathers (src/eField.cpp:204-316,
 // src/hField.cpp:159-267) in cell units.  "lo" family (E_r, H_z, H_phi):
@@ -147,12 +155,12 @@ struct GlobalFields {
   }
   __device__ __forceinline__ void get(const F32Args &p, int, int, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
                                       float h[3]) const {
-    e[0] = gather(p, p.er, p.escale, rl, zo);
-    e[1] = gather(p, p.ephi, p.escale, ro, zo);
-    e[2] = gather(p, p.ez, p.escale, ro, zh);
-    h[0] = gather(p, p.hhr, p.bscale, ro, zh);
-    h[1] = gather(p, p.hhphi, p.bscale, rl, zh);
-    h[2] = gather(p, p.hhz, p.bscale, rl, zo);
+    e[0] = gather(p, field(p, kER), p.escale, rl, zo);
+    e[1] = gather(p, field(p, kEPhi), p.escale, ro, zo);
+    e[2] = gather(p, field(p, kEZ), p.escale, ro, zh);
+    h[0] = gather(p, field(p, kHHR), p.bscale, ro, zh);
+    h[1] = gather(p, field(p, kHHPhi), p.bscale, rl, zh);
+    h[2] = gather(p, field(p, kHHZ), p.bscale, rl, zo);
   }
 };
 
@@ -209,12 +217,12 @@ struct BoxFields {
       e[0] = e0; e[1] = e1; e[2] = e2;
       h[0] = h0; h[1] = h1; h[2] = h2;
     } else {
-      const float e0 = gather(p, p.er, p.escale, rl, zo);
-      const float e1 = gather(p, p.ephi, p.escale, ro, zo);
-      const float e2 = gather(p, p.ez, p.escale, ro, zh);
-      const float h0 = gather(p, p.hhr, p.bscale, ro, zh);
-      const float h1 = gather(p, p.hhphi, p.bscale, rl, zh);
-      const float h2 = gather(p, p.hhz, p.bscale, rl, zo);
+      const float e0 = gather(p, field(p, kER), p.escale, rl, zo);
+      const float e1 = gather(p, field(p, kEPhi), p.escale, ro, zo);
+      const float e2 = gather(p, field(p, kEZ), p.escale, ro, zh);
+      const float h0 = gather(p, field(p, kHHR), p.bscale, ro, zh);
+      const float h1 = gather(p, field(p, kHHPhi), p.bscale, rl, zh);
+      const float h2 = gather(p, field(p, kHHZ), p.bscale, rl, zo);
       e[0] = e0; e[1] = e1; e[2] = e2;
       h[0] = h0; h[1] = h1; h[2] = h2;
     }
@@ -230,7 +238,8 @@ __device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int
     const int r = min(max(bi + (int)(threadIdx.x >> 1), 0), p.nr - 1);
     reinterpret_cast<float4 *>(box + 2 * kFSize)[threadIdx.x] = p.rows[2 * r + (threadIdx.x & 1)];
   }
-  const double *src[kFComp] = {p.er, p.ephi, p.ez, p.hhr, p.hhphi, p.hhz};
+  const double *src[kFComp] = {field(p, kER), field(p, kEPhi), field(p, kEZ), field(p, kHHR), field(p, kHHPhi),
+                               field(p, kHHZ)};
   float2 *f = reinterpret_cast<float2 *>(box);
 #pragma unroll
   for (int comp = 0; comp < kFComp; comp++) {
@@ -430,9 +439,9 @@ __device__ __forceinline__ float cross_param(float f, float d) {
 // the old to the new position, split at the cell faces it crosses (at most
 // one radial and one axial face: |displacement| <= c dt <= dr/2), each piece
 // deposited in its own cell
-template <class S>
-__device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, int ko, float bo, int in_, float an,
-                          int kn, float bn, float4 rt) {
+template <class S, class FS>
+__device__ void deposit_j(const F32Args &p, S &sk, const FS &fs, float w, int io, float ao, int ko, float bo, int in_,
+                          float an, int kn, float bn, float4 rt) {
   int di = in_ - io, dk = kn - ko;
   if (di < -1 || di > 1 || dk < -1 || dk > 1) {
     atomicOr(p.err, kErrDomain);
@@ -459,7 +468,8 @@ __device__ void deposit_j(const F32Args &p, S &sk, float w, int io, float ao, in
         float fr = di > 0 ? 1.f : 0.f;
         segment(p, sk, w, ci, ck, ca, cb, fr - ca, sd_z, rt);
         ci += di;
-        rt = p.rows[2 * min(max(ci, 0), p.nr - 1) + 1];
+        float4 rt0;
+        fs.row(p, ci, rt0, rt);  // the field source's copy of the table (LDS in a tile)
         ca = di > 0 ? 0.f : 1.f;
         cb += sd_z;
         tr = 2.f;
@@ -624,7 +634,7 @@ __device__ __forceinline__ bool step_state32(const F32Args &p, S &sk, const FS &
   renorm(i1, xa2);
   renorm(k1, b2);
   if (alive) {
-    deposit_j(p, sk, w, i, a, k, b, i1, xa2, k1, b2, t1);
+    deposit_j(p, sk, fs, w, i, a, k, b, i1, xa2, k1, b2, t1);
     // back_velocity_to_rz with the second half step's angle (:1005-1023)
     float u0 = ct * u[0] - st * u[1];
     float u1 = st * u[0] + ct * u[1];
@@ -731,7 +741,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
   __shared__ double box[TileSink::kSize];
   __shared__ __align__(16) float fbox[kFBoxFloats];
   constexpr long long kTile = (long long)WAVES * PPT / kPPT * kChunk;
-  fused32_tile<PPT, WAVES>(p, gs, (long long)blockIdx.x * kTile, n_cap, box, fbox);
+  // launched for the chunked layout only, i.e. plasma species: no bunch walls
+  // and no charge density (constants here drop their code and registers)
+  F32Args q = p;
+  q.is_bunch = 0;
+  q.want_rho = 0;
+  // the sink's grid size and error word are the arguments' (one copy each in
+  // scalar registers)
+  GlobalSink g = gs;
+  g.nr = q.nr;
+  g.nz = q.nz;
+  g.err = q.err;
+  fused32_tile<PPT, WAVES>(q, g, (long long)blockIdx.x * kTile, n_cap, box, fbox);
 }
 
 // bunches (plain layout): the same walk, the box anchored at the smallest
@@ -805,19 +826,19 @@ struct F32ArgsG {
   int nr, nz;
   float hr, hz, inv_c2, c1, bscale, escale, kz, kr, kp;
   int pusher, is_bunch, want_rho, n;
-  gptr<const double> er, ephi, ez, hhr, hhphi, hhz;
+  gptr<const double> grid;
   gptr<const float4> rows;
   gptr<int> err;
   gptr<uint32_t> cell;
   gptr<float> fa, fb, ur, uphi, uz;
   gptr<const float> w;
 };
-static_assert(sizeof(F32ArgsG) == sizeof(F32Args) && offsetof(F32ArgsG, er) == offsetof(F32Args, er) &&
+static_assert(sizeof(F32ArgsG) == sizeof(F32Args) && offsetof(F32ArgsG, grid) == offsetof(F32Args, grid) &&
                   offsetof(F32ArgsG, w) == offsetof(F32Args, w) && offsetof(F32ArgsG, n) == offsetof(F32Args, n),
               "F32ArgsG mirrors F32Args");
 struct GlobalSinkG {
   int nr, nz;
-  gptr<double> jr, jphi, jz, rho;
+  gptr<double> j, rho;
   gptr<int> err;
 };
 static_assert(sizeof(GlobalSinkG) == sizeof(GlobalSink) && offsetof(GlobalSinkG, err) == offsetof(GlobalSink, err),
@@ -840,12 +861,7 @@ __device__ __forceinline__ F32Args from_global(const F32ArgsG &g) {
   a.is_bunch = g.is_bunch;
   a.want_rho = g.want_rho;
   a.n = g.n;
-  a.er = (const double *)g.er;
-  a.ephi = (const double *)g.ephi;
-  a.ez = (const double *)g.ez;
-  a.hhr = (const double *)g.hhr;
-  a.hhphi = (const double *)g.hhphi;
-  a.hhz = (const double *)g.hhz;
+  a.grid = (const double *)g.grid;
   a.rows = (const float4 *)g.rows;
   a.err = (int *)g.err;
   a.cell = (uint32_t *)g.cell;
@@ -861,9 +877,7 @@ __device__ __forceinline__ GlobalSink from_global(const GlobalSinkG &g) {
   GlobalSink s;
   s.nr = g.nr;
   s.nz = g.nz;
-  s.jr = (double *)g.jr;
-  s.jphi = (double *)g.jphi;
-  s.jz = (double *)g.jz;
+  s.j = (double *)g.j;
   s.rho = (double *)g.rho;
   s.err = (int *)g.err;
   return s;
@@ -1024,12 +1038,7 @@ F32Args args32(Ctx &c, Species &sp) {
   p.is_bunch = sp.kind == 1;
   p.want_rho = sp.kind == 1;
   p.n = sp.n;
-  p.er = c.a[kER];
-  p.ephi = c.a[kEPhi];
-  p.ez = c.a[kEZ];
-  p.hhr = c.a[kHHR];
-  p.hhphi = c.a[kHHPhi];
-  p.hhz = c.a[kHHZ];
+  p.grid = c.grid;
   p.rows = c.rows;
   p.err = c.err;
   p.cell = (uint32_t *)sp.f32.cell;
@@ -1046,9 +1055,7 @@ GlobalSink sink32(Ctx &c) {
   GlobalSink gs;
   gs.nr = c.g.nr;
   gs.nz = c.g.nz;
-  gs.jr = c.a[kJR];
-  gs.jphi = c.a[kJPhi];
-  gs.jz = c.a[kJZ];
+  gs.j = c.a[kJR];
   gs.rho = c.a[kRhoBunch];
   gs.err = c.err;
   return gs;

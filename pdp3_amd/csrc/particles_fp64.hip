@@ -386,7 +386,8 @@ __device__ void strict_motion(const GridDesc &g, S &acc, double q, double radius
       r0 = (i_n + 0.5) * dr;
       ssd = density_strict(q, r0, dr, dz, delta_t);
       // `r0 * r0 / i_n * dr` is ((r0*r0)/i_n)*dr in the reference (:941)
-      wj = ssd * (delta_r - r0 * r0 / (i_n * dr + delta_r) + r0 * r0 / i_n * dr +
+      const double slip = g.strict_fix ? r0 * r0 / (i_n * dr) : r0 * r0 / i_n * dr;
+      wj = ssd * (delta_r - r0 * r0 / (i_n * dr + delta_r) + slip +
                   dr * dr / (4. * (i_n * dr + delta_r)) - dr * dr / (4. * i_n * dr));
       acc.template add<kCR>(i_n, k_n, wj * left_delta_z);
       acc.template add<kCR>(i_n, k_n + 1, wj * right_delta_z);
@@ -402,7 +403,8 @@ __device__ void strict_motion(const GridDesc &g, S &acc, double q, double radius
       r0 = (i_n + 0.5) * dr;
       ssd = density_strict(q, r0, dr, dz, delta_t);
       // same precedence as above (:976)
-      wj = ssd * (delta_r - r0 * r0 / ((i_n + 1) * dr + delta_r) + r0 * r0 / (i_n + 1) * dr +
+      const double slip1 = g.strict_fix ? r0 * r0 / ((i_n + 1) * dr) : r0 * r0 / (i_n + 1) * dr;
+      wj = ssd * (delta_r - r0 * r0 / ((i_n + 1) * dr + delta_r) + slip1 +
                   dr * dr / (4. * ((i_n + 1) * dr + delta_r)) - dr * dr / (4. * (i_n + 1) * dr));
       acc.template add<kCR>(i_n, k_n, wj * left_delta_z);
       acc.template add<kCR>(i_n, k_n + 1, wj * right_delta_z);
@@ -831,9 +833,7 @@ GlobalSink sink_of(Ctx &c) {
   GlobalSink gs;
   gs.nr = c.g.nr;
   gs.nz = c.g.nz;
-  gs.jr = c.a[kJR];
-  gs.jphi = c.a[kJPhi];
-  gs.jz = c.a[kJZ];
+  gs.j = c.a[kJR];
   gs.rho = c.a[kRhoBunch];
   gs.err = c.err;
   return gs;

@@ -22,12 +22,15 @@ namespace pdp3 {
 
 enum SinkComp { kCR = 0, kCPhi = 1, kCZ = 2, kCRho = 3 };
 
+// j: J_r of the context's grid, with J_phi and J_z the next nr*nz arrays
+// (one base pointer instead of three keeps the particle kernels' scalar
+// registers from spilling); rho: the bunch charge density, or a caller's array
 struct GlobalSink {
   int nr, nz;
-  double *jr, *jphi, *jz, *rho;
+  double *j, *rho;
   int *err;
   __device__ __forceinline__ double *base(int comp) const {
-    return comp == kCR ? jr : comp == kCPhi ? jphi : comp == kCZ ? jz : rho;
+    return comp == kCRho ? rho : j + (size_t)comp * ((size_t)nr * (size_t)nz);
   }
   __device__ __forceinline__ void put(int comp, int i, int k, double v) const {
     if ((unsigned)i < (unsigned)nr && (unsigned)k < (unsigned)nz)

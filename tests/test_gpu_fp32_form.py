@@ -149,6 +149,45 @@ def test_face_crossing_below_fp32_resolution(axis):
         s.close()
 
 
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_strict_radial_crossing_precedence(direction):
+    """Purely radial moves (v_z = 0) of 0.01 cells across a radial face take
+    the reference's strict_motion_weighting, whose crossing branch parses
+    `r0 * r0 / i_n * dr` as ((r0^2)/i_n) dr (src/particles.cpp:941, :976).
+    With pdp3_set_strict_radial_fix the FP64 form evaluates r0^2/(i_n dr) and
+    its J_r matches the FP32 form's closed forms to 1e-5 normwise; without it
+    (the reference's arithmetic) J_r is larger by orders of magnitude."""
+    from pdp3_amd import capi
+
+    rng = np.random.default_rng(33)
+    nr, nz, dr = 33, 129, 5e-4
+    dt = 0.5 * dr / 3e8
+    n = 512
+    i = rng.integers(3, 29, n)
+    k = rng.integers(3, 125, n)
+    off = 0.005 if direction < 0 else 0.995
+    r = (i + off) * dr
+    z = (k + 0.5) * dr
+    v = np.zeros((3, n))
+    v[0] = direction * 0.01 * dr / dt
+    w = np.full(n, 1e6)
+    plain = capi.Simulation(nr, nz, 32 * dr, 128 * dr, dt, precision=capi.PREC_FP64)
+    fixed = capi.Simulation(nr, nz, 32 * dr, 128 * dr, dt, precision=capi.PREC_FP64)
+    fixed.set_strict_radial_fix(True)
+    fp32 = capi.Simulation(nr, nz, 32 * dr, 128 * dr, dt, precision=capi.PREC_FP32)
+    for s in (plain, fixed, fp32):
+        s.add_plasma(-1, 1.0, r, z, v[0], v[1], v[2], w)
+        s.step(1)
+    jp, jf, j32 = (s.field("J_r") for s in (plain, fixed, fp32))
+    e_fix = normwise(j32, jf)
+    ratio = np.abs(jp).max() / np.abs(jf).max()
+    print("direction", direction, "FP32 vs FP64 (fixed)", e_fix, "plain / fixed max", ratio)
+    assert e_fix <= 1e-5, e_fix
+    assert ratio >= 10, ratio
+    for s in (plain, fixed, fp32):
+        s.close()
+
+
 @pytest.mark.parametrize("pusher", [0, 1])
 def test_cfg1_100_steps_error(pusher, orc):
     from pdp3_amd import capi
