@@ -895,7 +895,12 @@ k_ens_fused32(const F32Args *__restrict__ args, const GlobalSink *__restrict__ s
     const EnsItem item = items[it];
     const F32ArgsG *ag = reinterpret_cast<const F32ArgsG *>(args) + item.arg;
     const GlobalSinkG *sg = reinterpret_cast<const GlobalSinkG *>(sinks) + item.arg;
-    fused32_tile<PPT, WAVES>(from_global(*ag), from_global(*sg), item.base, item.n_cap, box, fbox);
+    const F32Args a = from_global(*ag);
+    GlobalSink g = from_global(*sg);
+    g.nr = a.nr;  // one copy of the grid size and error word (scalar registers)
+    g.nz = a.nz;
+    g.err = a.err;
+    fused32_tile<PPT, WAVES>(a, g, item.base, item.n_cap, box, fbox);
     __syncthreads();  // the next item clears the boxes
   }
 }
