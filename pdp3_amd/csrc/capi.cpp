@@ -1579,6 +1579,8 @@ struct pdp3_ensemble {
   int n_inj = 0, inj_max = 0;
   int *inj_tables = nullptr;
   long long *step_dev = nullptr;
+  // the periodic sorts: every instance's plasma in one launch sequence
+  MultiSort ms;
 };
 
 namespace {
@@ -1774,6 +1776,7 @@ int pdp3_ensemble_destroy(pdp3_ensemble *e) {
   if (e->inj) (void)hipFree(e->inj);
   if (e->inj_tables) (void)hipFree(e->inj_tables);
   if (e->step_dev) (void)hipFree(e->step_dev);
+  sort_multi_free(e->ms);
   if (e->ev0) (void)hipEventDestroy(e->ev0);
   if (e->ev1) (void)hipEventDestroy(e->ev1);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1793,17 +1796,13 @@ int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps) {
   if (!e || nsteps < 0) return PDP3_ERR_BAD_ARG;
   int st;
   bool timing = false;
-  if (e->sort_interval > 0)
-    for (pdp3_ctx *ctx : e->ctxs)
-      if ((st = prepare_sorts(ctx))) return efail(e, st, ctx->c.last_error);
   for (int s = 0; s < nsteps; s++) {
     if (e->sort_interval > 0 && e->steps_done > 0 && e->steps_done % e->sort_interval == 0) {
+      // every instance's plasma in one sort on the ensemble's stream; the
+      // previous step has finished before its particle tables are rewritten
       ENS_HIP(e, hipStreamSynchronize(e->stream));
-      // every instance's sort is enqueued on its own stream before any is waited for
-      for (pdp3_ctx *ctx : e->ctxs)
-        if ((st = sort_plasma(ctx, false))) return efail(e, st, ctx->c.last_error);
-      for (pdp3_ctx *ctx : e->ctxs)
-        if ((st = pdp3_sync(ctx))) return efail(e, st, ctx->c.last_error);
+      st = sort_multi(e->ms, e->cs, e->stream);
+      if (st) return efail(e, st == -3 ? PDP3_ERR_OOM : PDP3_ERR_HIP, "ensemble sort failed");
       if (ens_particles_update(e->parts, e->cs) != 0) return efail(e, PDP3_ERR_HIP, "ensemble tables");
     }
     if ((st = ens_graph(e))) return st;

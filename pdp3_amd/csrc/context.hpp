@@ -286,6 +286,32 @@ int sort_prepare(Ctx &c, Species &sp);
 // workspace bytes the periodic sorts of the context's plasma species need
 // (their current record space), and a shrink of the shared workspace back to
 // that size after an exact sort grew it (waits for the stream; 0 or -3)
+// several plasma species (an ensemble's) sorted as one: the segment table
+// the kernels read (global slots [slot_base, slot_base + n_cap), keys from
+// key_base), and the state the multi-segment sort keeps between sorts
+struct MultiSeg {
+  const uint32_t *cell;
+  long long slot_base;
+  uint32_t key_base, ncell, nzm1;
+  void *in[7], *out[7];
+};
+struct MultiSort {
+  MultiSeg *seg = nullptr;             // device table
+  uint32_t *seg_of_chunk = nullptr;    // device, one entry per global chunk
+  long long n = 0, nseg = 0, rec_cap = 0;
+  void *ws = nullptr;
+  size_t ws_bytes = 0;
+  uint32_t *runs_host = nullptr;       // pinned: the previous sort's run count
+  hipEvent_t runs_ev = nullptr;
+  bool counted = false;
+  std::vector<MultiSeg> seg_host;      // the table's source (kept for the asynchronous copy)
+};
+// sort the plasma species of the given contexts in one launch sequence on
+// `stream` (same result as sorting each with sort_species, exact = false);
+// swaps each species' arrays with its second buffer.  0, -1 (a species not
+// in whole chunks), -3 (allocation or HIP error)
+int sort_multi(MultiSort &ms, const std::vector<Ctx *> &cs, hipStream_t stream);
+void sort_multi_free(MultiSort &ms);
 size_t sort_periodic_bytes(const Ctx &c);
 int sort_shrink_ws(Ctx &c, size_t need);
 

@@ -191,10 +191,14 @@ __device__ __forceinline__ void chunk_keys(const K &keys, int nc, uint32_t *t) {
   const long long c0 = (long long)blockIdx.x * kCPB * kChunk;
   uint32_t v[kCPB * kGroups];
 #pragma unroll
-  for (int c = 0; c < kCPB; c++)
+  for (int c = 0; c < kCPB; c++) {
+    // the key function of this chunk (uniform over the block; a chunk past
+    // the species' last takes the block's first, whose keys are not used)
+    const auto kc = keys.at_chunk(c < nc ? c0 + (long long)c * kChunk : c0);
 #pragma unroll
     for (int m = 0; m < kGroups; m++)
-      v[c * kGroups + m] = c < nc ? keys(c0 + (long long)c * kChunk + m * 256 + (int)threadIdx.x) : 0u;
+      v[c * kGroups + m] = c < nc ? kc(c0 + (long long)c * kChunk + m * 256 + (int)threadIdx.x) : 0u;
+  }
 #pragma unroll
   for (int c = 0; c < kCPB; c++)
 #pragma unroll
@@ -374,6 +378,22 @@ __global__ void __launch_bounds__(256) k_gather_f64(Arrays7 in, Arrays7 out, con
   gather_array<uint8_t>(in.a[6], out.a[6], q, c0 + threadIdx.x);
 }
 
+// the multi-segment gather: one chunk per block, the chunk's segment gives
+// the arrays, the permutation holds global slots of the same segment
+__global__ void __launch_bounds__(256) k_gather_f32_multi(const MultiSeg *__restrict__ seg,
+                                                          const uint32_t *__restrict__ seg_of_chunk,
+                                                          const uint32_t *__restrict__ src) {
+  const long long c0 = (long long)blockIdx.x * kChunk;
+  const MultiSeg &d = seg[seg_of_chunk[blockIdx.x]];
+  uint32_t q[kGatherPer];
+  gather_sources(src, c0, q);
+#pragma unroll
+  for (int u = 0; u < kGatherPer; u++) q[u] -= (uint32_t)d.slot_base;
+  const long long p0 = c0 - d.slot_base + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 7; k++) gather_array<uint32_t>(d.in[k], d.out[k], q, p0);
+}
+
 // the sort key of the particle in slot j, from its state
 struct KeysF32 {
   GridDesc g;
@@ -385,6 +405,7 @@ struct KeysF32 {
     if (c & 0x80000000u) return dead_key;
     return ((c >> 16) & 0x7FFF) * (uint32_t)(g.nz - 1) + (c & 0xFFFF);
   }
+  __device__ __forceinline__ const KeysF32 &at_chunk(long long) const { return *this; }
 };
 struct KeysF64 {
   GridDesc g;
@@ -399,6 +420,37 @@ struct KeysF64 {
     k = max(0, min(k, g.nz - 2));
     return (uint32_t)i * (uint32_t)(g.nz - 1) + (uint32_t)k;
   }
+  __device__ __forceinline__ const KeysF64 &at_chunk(long long) const { return *this; }
+};
+
+// several species sorted as one: segment s holds global slots
+// [slot_base, slot_base + n_cap) (whole chunks) and keys
+// [key_base, key_base + ncell + 2), so the sorted order keeps every
+// particle inside its own segment
+__device__ __forceinline__ int seg_of(const MultiSeg *seg, const uint32_t *seg_of_chunk, long long j) {
+  return (int)seg_of_chunk[j / kChunk];
+}
+// the keys of one segment (global slot j)
+struct KeysSeg {
+  const uint32_t *cell;  // shifted so that cell[j] is global slot j
+  uint32_t key_base, ncell, nzm1;
+  __device__ __forceinline__ uint32_t operator()(long long j) const {
+    const uint32_t c = cell[j];
+    if (c == kPadCell) return key_base + ncell + 1;
+    if (c & 0x80000000u) return key_base + ncell;
+    return key_base + ((c >> 16) & 0x7FFF) * nzm1 + (c & 0xFFFF);
+  }
+};
+struct KeysMulti {
+  const MultiSeg *seg;
+  const uint32_t *seg_of_chunk;
+  // the segment of the chunk starting at global slot j0
+  __device__ __forceinline__ KeysSeg at_chunk(long long j0) const {
+    const int s = __builtin_amdgcn_readfirstlane(seg_of(seg, seg_of_chunk, j0));
+    const MultiSeg &d = seg[s];
+    return KeysSeg{d.cell - d.slot_base, d.key_base, d.ncell, d.nzm1};
+  }
+  __device__ __forceinline__ uint32_t operator()(long long j) const { return at_chunk(j - j % kChunk)(j); }
 };
 
 void exclusive_scan(uint32_t *data, long long n, uint32_t *tmp, hipStream_t s) {
@@ -623,6 +675,134 @@ int sort_species(Ctx &c, Species &sp, bool exact) {
       out.a[k] = *bp[k];
     }
     hipLaunchKernelGGL(k_gather_f64, dim3(chunks), dim3(256), 0, c.stream, in, out, src);
+    for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
+  }
+  return 0;
+}
+
+
+void sort_multi_free(MultiSort &ms) {
+  if (ms.seg) (void)hipFree(ms.seg);
+  if (ms.seg_of_chunk) (void)hipFree(ms.seg_of_chunk);
+  if (ms.ws) (void)hipFree(ms.ws);
+  if (ms.runs_host) (void)hipHostFree(ms.runs_host);
+  if (ms.runs_ev) (void)hipEventDestroy(ms.runs_ev);
+  ms = MultiSort();
+}
+
+int sort_multi(MultiSort &ms, const std::vector<Ctx *> &cs, hipStream_t stream) {
+  // the segments: every sortable plasma species of every context, in order
+  std::vector<std::pair<Ctx *, Species *>> list;
+  for (Ctx *c : cs)
+    for (Species &sp : c->species)
+      if (sp.kind == 0 && sortable(*c, sp) && c->precision == 1) list.emplace_back(c, &sp);
+  if (list.empty()) return 0;
+  std::vector<MultiSeg> segs(list.size());
+  long long n = 0, keys = 0;
+  for (size_t s = 0; s < list.size(); s++) {
+    Ctx &c = *list[s].first;
+    Species &sp = *list[s].second;
+    if (sp.n_cap % kChunk != 0) return -1;
+    if (!ensure_alts(c, sp)) return -3;
+    MultiSeg &d = segs[s];
+    d.cell = (const uint32_t *)sp.f32.cell;
+    d.slot_base = n;
+    d.key_base = (uint32_t)keys;
+    d.ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
+    d.nzm1 = (uint32_t)(c.g.nz - 1);
+    SpeciesF32 &a = sp.f32, &b = sp.f32_alt;
+    void *ap[7] = {a.cell, a.fr, a.fz, a.vr, a.vphi, a.vz, a.w};
+    void *bp[7] = {b.cell, b.fr, b.fz, b.vr, b.vphi, b.vz, b.w};
+    for (int k = 0; k < 7; k++) {
+      d.in[k] = ap[k];
+      d.out[k] = bp[k];
+    }
+    n += sp.n_cap;
+    keys += (long long)d.ncell + 2;
+  }
+  if (n >= (1ll << 32) || keys >= (1ll << 32)) return -1;  // slots and keys are 32-bit
+  const long long nch = n / kChunk;
+  // the tables (rebuilt only when the segments change size)
+  if (ms.n != n || ms.nseg != (long long)list.size()) {
+    if (ms.seg) (void)hipFree(ms.seg);
+    if (ms.seg_of_chunk) (void)hipFree(ms.seg_of_chunk);
+    ms.seg = nullptr;
+    ms.seg_of_chunk = nullptr;
+    if (hipMalloc((void **)&ms.seg, list.size() * sizeof(MultiSeg)) != hipSuccess ||
+        hipMalloc((void **)&ms.seg_of_chunk, (size_t)nch * sizeof(uint32_t)) != hipSuccess)
+      return -3;
+    std::vector<uint32_t> soc((size_t)nch);
+    for (size_t s = 0; s < list.size(); s++) {
+      const long long c0 = segs[s].slot_base / kChunk, c1 = c0 + list[s].second->n_cap / kChunk;
+      for (long long ch = c0; ch < c1; ch++) soc[(size_t)ch] = (uint32_t)s;
+    }
+    if (hipMemcpy(ms.seg_of_chunk, soc.data(), soc.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return -3;
+    ms.n = n;
+    ms.nseg = (long long)list.size();
+    ms.rec_cap = 0;
+    ms.counted = false;
+  }
+  // the segment table changes with every sort (the arrays swap); the
+  // previous copy from seg_host has completed (the last sort's run-count
+  // event, waited for below or above, comes after it on the stream)
+  if (ms.counted && hipEventSynchronize(ms.runs_ev) != hipSuccess) return -3;
+  ms.seg_host = segs;
+  if (hipMemcpyAsync(ms.seg, ms.seg_host.data(), segs.size() * sizeof(MultiSeg), hipMemcpyHostToDevice, stream) !=
+      hipSuccess)
+    return -3;
+  int bits = 1;
+  while ((1ull << bits) <= (unsigned long long)keys) bits++;
+  // record capacity from the previous sort's run count (read back
+  // asynchronously), as sort_species does
+  if (!ms.runs_host) {
+    if (hipHostMalloc((void **)&ms.runs_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) return -3;
+    *ms.runs_host = 0;
+    if (hipEventCreateWithFlags(&ms.runs_ev, hipEventDisableTiming) != hipSuccess) return -3;
+  }
+  const long long last = ms.counted ? (long long)*ms.runs_host : 0;
+  long long cap = ms.rec_cap > 0 ? ms.rec_cap : std::min(n, n / 4 + 65536);
+  if (4 * last > 3 * cap) cap = std::min(n, 2 * last + 65536);
+  ms.rec_cap = cap;
+  const SortWs ws = sort_ws(n, cap);
+  if (ms.ws_bytes < ws.bytes) {
+    if (ms.ws) (void)hipFree(ms.ws);
+    ms.ws = nullptr;
+    ms.ws_bytes = 0;
+    if (hipMalloc(&ms.ws, ws.bytes) != hipSuccess) return -3;
+    ms.ws_bytes = ws.bytes;
+  }
+  const long long nb = ws.nb, rec_hist = ws.rec_hist, scratch = ws.scratch;
+  uint32_t *src = (uint32_t *)ms.ws, *block = src + n, *tmp = block + nb + 1;
+  uint32_t *rk = tmp + scratch, *ri = rk + cap, *rk2 = ri + cap, *ri2 = rk2 + cap, *rlen = ri2 + cap;
+  const uint32_t *count = block + nb;
+  const KeysMulti km{ms.seg, ms.seg_of_chunk};
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, stream, block + nb, 1LL, 0u);
+  const unsigned chunks = (unsigned)nch, rblocks = (unsigned)((nch + kCPB - 1) / kCPB);
+  hipLaunchKernelGGL(k_run_count<KeysMulti>, dim3(rblocks), dim3(256), 0, stream, km, nch, block);
+  exclusive_scan(block, nb + 1, tmp, stream);
+  hipLaunchKernelGGL(k_run_emit<KeysMulti>, dim3(rblocks), dim3(256), 0, stream, km, nch, block, count, cap, rk, ri,
+                     rlen);
+  radix_pairs(rk, ri, rk2, ri2, count, cap, bits, tmp, tmp + rec_hist, stream);
+  const unsigned mb = (unsigned)((cap + 255) / 256);
+  uint32_t *lens = rk2;
+  hipLaunchKernelGGL(k_run_lens, dim3(mb), dim3(256), 0, stream, ri, rlen, count, cap, lens);
+  exclusive_scan(lens, cap, tmp, stream);
+  hipLaunchKernelGGL(k_run_first, dim3(mb), dim3(256), 0, stream, ri, lens, count, cap, rlen);
+  hipLaunchKernelGGL(k_run_place<KeysMulti>, dim3(rblocks), dim3(256), 0, stream, km, nch, block, count, cap, rlen,
+                     src);
+  if (hipMemcpyAsync(ms.runs_host, count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipEventRecord(ms.runs_ev, stream) != hipSuccess)
+    return -3;
+  ms.counted = true;
+  hipLaunchKernelGGL(k_gather_f32_multi, dim3(chunks), dim3(256), 0, stream, ms.seg, ms.seg_of_chunk, src);
+  if (hipGetLastError() != hipSuccess) return -3;
+  for (auto &e : list) {
+    SpeciesF32 &a = e.second->f32, &b = e.second->f32_alt;
+    void **ap[7] = {(void **)&a.cell, (void **)&a.fr, (void **)&a.fz, (void **)&a.vr, (void **)&a.vphi,
+                    (void **)&a.vz, (void **)&a.w};
+    void **bp[7] = {(void **)&b.cell, (void **)&b.fr, (void **)&b.fz, (void **)&b.vr, (void **)&b.vphi,
+                    (void **)&b.vz, (void **)&b.w};
     for (int k = 0; k < 7; k++) std::swap(*ap[k], *bp[k]);
   }
   return 0;
