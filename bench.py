@@ -288,7 +288,66 @@ def bench_ensemble(a):
     ens.close()
     for s_ in sims:
         s_.close()
+    err_steps = 1000 if a.error_steps is None else a.error_steps
+    if err_steps > 0:
+        out["error_fp32_vs_fp64"] = ensemble_error(wl, capi, insts, err_steps, not a.loaded_bunches)
+        print(f"# cfg4 error: {out['error_fp32_vs_fp64']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
+
+
+def ensemble_error(wl, capi, insts, steps, injected, sample=8):
+    """E_z and momentum errors of the FP32 form against the FP64 plain form
+    (strict-radial precedence fix, as in ez_error_full) after `steps` steps,
+    on a sample of the ensemble's members, each run alone (an ensemble member
+    equals its single run: tests/test_gpu_ensemble.py), no cell sort so that
+    particles pair up by index.  Reports the largest figure over the sample."""
+    rng = np.random.default_rng(11)
+    picks = sorted(rng.choice(len(insts), size=min(sample, len(insts)), replace=False).tolist())
+    worst = dict(E_z_rel_rms=0.0, E_z_rel_max=0.0, momentum_rel_rms=0.0, momentum_rel_max=0.0,
+                 fp64_perturbed_1e_7_E_z_rel_rms=0.0)
+    per = []
+    for j in picks:
+        ez, mom = [], []
+        for run, p in enumerate((capi.PREC_FP32, capi.PREC_FP64, capi.PREC_FP64)):
+            sim = wl.build_instance(insts[j], precision=p, injected=injected)
+            if p == capi.PREC_FP64:
+                sim.set_strict_radial_fix(True)
+            if run == 2:
+                # the FP64 run again with every initial velocity component
+                # perturbed by 1e-7 relative (the size of one FP32 rounding):
+                # how far the loop itself carries such a change
+                st = sim.download(0)
+                prng = np.random.default_rng(5)
+                f = lambda: 1.0 + 1e-7 * prng.standard_normal(len(st["vr"]))
+                sim.upload(0, st["r"], st["z"], st["vr"] * f(), st["vphi"] * f(), st["vz"] * f(), st["w"], st["alive"])
+                del st
+            sim.step(steps)
+            ez.append(sim.field("E_z"))
+            d = sim.download(0)
+            g = 1.0 / np.sqrt(1.0 - (d["vr"] ** 2 + d["vphi"] ** 2 + d["vz"] ** 2) / wl.LIGHT_SPEED ** 2)
+            mom.append(np.stack([g * d["vr"], g * d["vphi"], g * d["vz"]]))
+            sim.close()
+        d_ez = ez[0] - ez[1]
+        d_p = mom[0] - mom[1]
+        fig = dict(E_z_rel_rms=float(np.sqrt(np.mean(d_ez ** 2) / np.mean(ez[1] ** 2))),
+                   E_z_rel_max=float(np.max(np.abs(d_ez)) / np.max(np.abs(ez[1]))),
+                   momentum_rel_rms=float(np.sqrt(np.sum(d_p ** 2) / np.sum(mom[1] ** 2))),
+                   momentum_rel_max=float(np.max(np.abs(d_p)) / np.max(np.abs(mom[1]))))
+        d64 = ez[2] - ez[1]
+        fig["fp64_perturbed_1e-7_E_z_rel_rms"] = float(np.sqrt(np.mean(d64 ** 2) / np.mean(ez[1] ** 2)))
+        fig["fp64_perturbed_1e_7_E_z_rel_rms"] = fig.pop("fp64_perturbed_1e-7_E_z_rel_rms")
+        for k in worst:
+            worst[k] = max(worst[k], fig[k])
+        i = insts[j]
+        per.append(dict(member=j, cells=[i.cells_r, i.cells_z], ppc=i.ppc, density=i.density,
+                        bunch_ratio=round(i.bunch_ratio, 3), **{k: float("%.3g" % v) for k, v in fig.items()}))
+    worst["per_member"] = per
+    worst["after_steps"] = steps
+    worst["reference"] = "FP64 form with the strict-radial precedence fix"
+    worst["sample"] = (f"members {picks} of {len(insts)}, each run alone, no cell sort; the largest figure "
+                       f"over the sample; fp64_perturbed_1e_7: the FP64 run against itself with the initial "
+                       f"velocities perturbed by 1e-7 relative once")
+    return worst
 
 
 def bench_adjoint(a):
@@ -316,6 +375,16 @@ def bench_adjoint(a):
     t2 = time.perf_counter()
     log(f"gradient: {t2 - t1:.2f} s")
     g = theta.grad.numpy()
+    # the same gradient with the FP64 forward (the reference's arithmetic)
+    theta64 = th.clone().requires_grad_(True)
+    phi64 = adj.PicObjective.apply(theta64, prob, capi.PREC_FP64)
+    phi64.backward()
+    g64 = theta64.grad.numpy()
+    err = {"phi_rel": abs(float(phi) - float(phi64)) / abs(float(phi64)),
+           "gradient_normwise": float(np.max(np.abs(g - g64)) / np.max(np.abs(g64))),
+           "gradient_max_component_rel": float(np.max(np.abs(g - g64) / np.abs(g64))),
+           "reference": "the same adjoint with the FP64 plain-form forward"}
+    log(f"FP32 forward against FP64 forward: {err}")
     fd_idx = [int(x) for x in a.fd.split(",")] if a.fd else []
     fd = None
     if fd_idx:
@@ -340,6 +409,7 @@ def bench_adjoint(a):
                    spec.ppc * spec.cells_r * spec.cells_z + spec.bunch_macro},
         "forward_only_s": t1 - t0, "phi": float(phi), "phi_forward_only": phi_fwd,
         "gradient": g.tolist(),
+        "error_fp32_vs_fp64": err,
     }
     if fd is not None:
         out["fd_fp64"] = {str(j): fd[j] for j in fd_idx}
