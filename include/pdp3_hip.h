@@ -327,10 +327,12 @@ int pdp3_objective_run(pdp3_ctx *ctx, int nsteps, int window_steps, double z_lo,
  * Independent instances (contexts set up with the calls above: any grid,
  * PML, plasma, loaded bunches; FP32 form, one common delta_t) advanced
  * together: per step one graph replay with an H launch, a J / rho reset, ONE
- * persistent particle launch over the plasma tiles and bunch blocks of every
- * instance, and an E launch.  The contexts stay owned by the caller and must
- * not be stepped on their own while the ensemble lives; their clocks advance
- * with the ensemble.  Bunches must be loaded (pdp3_bunch_add_loaded). */
+ * particle launch over the plasma tiles and bunch blocks of every instance,
+ * and an E launch.  The contexts stay owned by the caller and must not be
+ * stepped on their own while the ensemble lives; their clocks advance with
+ * the ensemble.  Bunches may be loaded (pdp3_bunch_add_loaded) or injected
+ * (pdp3_bunch_add): the injections run inside the graph from a device step
+ * counter. */
 typedef struct pdp3_ensemble pdp3_ensemble;
 int pdp3_ensemble_create(pdp3_ctx *const *ctxs, int n, pdp3_ensemble **out);
 int pdp3_ensemble_destroy(pdp3_ensemble *e);
@@ -343,7 +345,7 @@ int pdp3_ensemble_step(pdp3_ensemble *e, int nsteps);
 int pdp3_ensemble_set_sort_interval(pdp3_ensemble *e, int sort_interval);
 /* device time of the stepped graphs since the last call (HIP events), and the step count */
 int pdp3_ensemble_timing(pdp3_ensemble *e, double *total_ms, long long *steps);
-/* tile shape of the persistent particle kernel, as pdp3_set_tile; the
+/* tile shape of the ensemble's particle kernel, as pdp3_set_tile; the
  * autotuner times steps_per_candidate real steps per shape and keeps the fastest */
 int pdp3_ensemble_set_tile(pdp3_ensemble *e, int ppt, int waves);
 int pdp3_ensemble_get_tile(const pdp3_ensemble *e, int *ppt, int *waves);

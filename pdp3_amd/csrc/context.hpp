@@ -190,7 +190,7 @@ void ens_fields_free(EnsFields *e);
 void ens_launch_h(EnsFields *e, hipStream_t st);
 void ens_launch_e(EnsFields *e, hipStream_t st);
 void ens_launch_zero(EnsFields *e, hipStream_t st);
-// the plasma tiles and bunch blocks of every instance, one persistent launch;
+// the plasma tiles and bunch blocks of every instance, one launch;
 // rebuild after a sort (particle arrays move)
 struct EnsParticles;
 EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs);

@@ -164,14 +164,17 @@ struct GlobalFields {
   }
 };
 
-// rows bi..bi+kFR-1, columns bk..bk+kFK-1 of the six components, anchored one
-// row and one column below the block's first cell (ai, ak): a particle in cell
-// (i, k) reads rows i-1..i+1 and columns k-1..k+1, so every cell with
-// i in [ai, ai+1] and k in [ak, ak+kFK-3] is served from the box.  Each entry
-// is a pair (node k, node k+1) of its row, so that the four nodes of one
-// component's stencil are two 8-byte entries kFK apart (one LDS instruction)
-// and the six components differ only by a constant offset.
-constexpr int kFR = 4, kFK = 64, kFComp = 6, kFSize = kFComp * kFR * kFK;
+// rows bi..bi+kFR-1, columns bk..bk+kFK-1 of the six components, anchored two
+// rows and two columns below the tile's anchor cell (ai, ak): a particle in
+// cell (i, k) reads rows i-1..i+1 and columns k-1..k+1, so every cell with
+// i in [ai-1, ai+1] and k in [ak-1, ak+kFK-4] is served from the box (a
+// sorted tile lies along row ai; between sorts some of its particles have
+// moved to the rows next to it, or to the column before its first).  Each
+// entry is a pair (node k, node k+1) of its row, so that the four nodes of
+// one component's stencil are two 8-byte entries kFK apart (one LDS
+// instruction) and the six components differ only by a constant offset.
+constexpr int kFR = 5, kFK = 64, kFComp = 6, kFSize = kFComp * kFR * kFK;
+constexpr int kFBelow = 2;  // box rows and columns below the anchor cell
 // the LDS array (floats): the field box (two floats per entry), then two
 // float4 row entries for each box row
 constexpr int kFBoxFloats = 2 * kFSize + 8 * kFR;
@@ -183,14 +186,14 @@ __device__ __forceinline__ float gather_box(const float2 *e, RadW rw, AxW aw) {
 }
 
 struct BoxFields {
-  const float *box;  // LDS, kFComp x kFR x kFK pairs, then the row entries of rows ai-1 .. ai+2
-  int ai, ak;        // first cell of the block
+  const float *box;  // LDS, kFComp x kFR x kFK pairs, then the row entries of rows ai-2 .. ai+2
+  int ai, ak;        // anchor cell of the tile
   // the box's row entries (an LDS read for every particle), replaced by the
   // entries evaluated in place for a row outside the box.  A global load here
   // would make the read a flat load, whose wait also waits for the next
   // particle's state loads already in flight.
   __device__ __forceinline__ void row(const F32Args &p, int i, float4 &t0, float4 &t1) const {
-    const int li = i - ai + 1;
+    const int li = i - ai + kFBelow;
     const int lc = min(max(li, 0), kFR - 1);
     const float4 *t = reinterpret_cast<const float4 *>(box + 2 * kFSize);
     t0 = t[2 * lc];
@@ -199,10 +202,10 @@ struct BoxFields {
   }
   __device__ __forceinline__ void get(const F32Args &p, int i, int k, RadW rl, RadW ro, AxW zo, AxW zh, float e[3],
                                       float h[3]) const {
-    const bool inbox =
-        (unsigned)(i - ai) < 2u && (unsigned)(k - ak) < (unsigned)(kFK - 2) && i <= p.nr - 2 && k <= p.nz - 2;
+    const bool inbox = (unsigned)(i - ai + kFBelow - 1) < (unsigned)(kFR - 2) &&
+                       (unsigned)(k - ak + kFBelow - 1) < (unsigned)(kFK - 2) && i <= p.nr - 2 && k <= p.nz - 2;
     if (inbox) {
-      const int bi = ai - 1, bk = ak - 1;
+      const int bi = ai - kFBelow, bk = ak - kFBelow;
       constexpr int C = kFR * kFK;
       const float2 *f = reinterpret_cast<const float2 *>(box);
       // the stencil's first entry for each (radial, axial) weight pair
@@ -233,7 +236,7 @@ struct BoxFields {
 // particle that the box serves reads them, nor the second half of a row's
 // last entry)
 __device__ __forceinline__ void field_box_fill(const F32Args &p, float *box, int ai, int ak) {
-  const int bi = ai - 1, bk = ak - 1;
+  const int bi = ai - kFBelow, bk = ak - kFBelow;
   if (threadIdx.x < 2 * kFR) {
     const int r = min(max(bi + (int)(threadIdx.x >> 1), 0), p.nr - 1);
     reinterpret_cast<float4 *>(box + 2 * kFSize)[threadIdx.x] = p.rows[2 * r + (threadIdx.x & 1)];
@@ -696,13 +699,12 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
                                              double *box, float *fbox) {
   static_assert(kFK == 64, "one wavefront per field box row");
   static_assert(kPPT % PPT == 0 && (WAVES * PPT) % kPPT == 0, "tile shape");
-  int ai = 0, ak = 0;
-  if (base < n_cap) {
-    uint32_t c0 = p.cell[base];
-    if (!(c0 & kDead)) {
-      ai = cell_i(c0);
-      ak = cell_k(c0);
-    }
+  // anchor samples: slot base + L is the logical particle 16 L of the tile
+  int ai, ak;
+  {
+    const int l = threadIdx.x & 63;
+    const uint32_t c = l < 8 && base + l < n_cap ? p.cell[base + l] : kDead;
+    tile_anchor(!(c & kDead), cell_i(c), cell_k(c), ai, ak);
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);
@@ -803,7 +805,8 @@ __global__ void __launch_bounds__(256) k_fused32_bunch(F32Args p, GlobalSink gs,
 }
 
 // ensembles: the plasma tiles and bunch blocks of every instance in one
-// persistent launch; blocks walk the item list with a stride of the grid.
+// launch, one block per item of the work list (a persistent grid striding
+// over the list kept the loop's state in scalar registers, which spilled).
 // Bunch blocks take the plasma tile path too (their particles are not cell
 // sorted, so most of their gathers and deposits take the boxes' HBM
 // fallbacks), which keeps the kernel at the plasma path's register count.
@@ -884,25 +887,25 @@ __device__ __forceinline__ GlobalSink from_global(const GlobalSinkG &g) {
 }
 
 template <int PPT, int WAVES>
-// 5 waves per SIMD (96 VGPRs, a few spilled) once the table pointers are
-// global; with flat pointers it needed 121 VGPRs and ran at 4
+// 5 waves per SIMD (96 VGPRs, 6 scalar registers spilled into vector lanes)
+// once the table pointers are global; with flat pointers it needed 121 VGPRs
+// and ran at 4
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_ens_fused32(const F32Args *__restrict__ args, const GlobalSink *__restrict__ sinks,
               const EnsItem *__restrict__ items, int n_items) {
   __shared__ double box[TileSink::kSize];
   __shared__ __align__(16) float fbox[kFBoxFloats];
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const EnsItem item = items[it];
-    const F32ArgsG *ag = reinterpret_cast<const F32ArgsG *>(args) + item.arg;
-    const GlobalSinkG *sg = reinterpret_cast<const GlobalSinkG *>(sinks) + item.arg;
-    const F32Args a = from_global(*ag);
-    GlobalSink g = from_global(*sg);
-    g.nr = a.nr;  // one copy of the grid size and error word (scalar registers)
-    g.nz = a.nz;
-    g.err = a.err;
-    fused32_tile<PPT, WAVES>(a, g, item.base, item.n_cap, box, fbox);
-    __syncthreads();  // the next item clears the boxes
-  }
+  const int it = blockIdx.x;
+  if (it >= n_items) return;
+  const EnsItem item = items[it];
+  const F32ArgsG *ag = reinterpret_cast<const F32ArgsG *>(args) + item.arg;
+  const GlobalSinkG *sg = reinterpret_cast<const GlobalSinkG *>(sinks) + item.arg;
+  const F32Args a = from_global(*ag);
+  GlobalSink g = from_global(*sg);
+  g.nr = a.nr;  // one copy of the grid size and error word (scalar registers)
+  g.nz = a.nz;
+  g.err = a.err;
+  fused32_tile<PPT, WAVES>(a, g, item.base, item.n_cap, box, fbox);
 }
 
 // plain layout (bunches): one particle per thread, deposits straight to HBM
@@ -1159,29 +1162,9 @@ struct EnsParticles {
   // work list per tile shape (plasma tiles and bunch blocks)
   EnsItem *items[kNumTiles] = {};
   int n_items[kNumTiles] = {};
-  int blocks[kNumTiles] = {};
 };
 
 namespace {
-template <int PPT, int WAVES>
-int ens_blocks() {
-  int per_cu = 0, cus = 0, dev = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ens_fused32<PPT, WAVES>, WAVES * 64, 0);
-  return std::max(1, per_cu) * std::max(1, cus);
-}
-int ens_blocks_of(int tile) {
-  switch (tile) {
-    case 0: return ens_blocks<16, 4>();
-    case 1: return ens_blocks<16, 2>();
-    case 2: return ens_blocks<16, 1>();
-    case 3: return ens_blocks<8, 4>();
-    case 4: return ens_blocks<8, 2>();
-    default: return ens_blocks<4, 4>();
-  }
-}
-
 // the argument table and the work lists of the ensemble's species
 void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<EnsItem> items[kNumTiles]) {
   int a = 0;
@@ -1235,7 +1218,6 @@ EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
     ok = hipMalloc((void **)&e->items[t], std::max<size_t>(1, items[t].size()) * sizeof(EnsItem)) == hipSuccess &&
          (items[t].empty() || hipMemcpy(e->items[t], items[t].data(), items[t].size() * sizeof(EnsItem),
                                         hipMemcpyHostToDevice) == hipSuccess);
-    e->blocks[t] = ens_blocks_of(t);
   }
   if (!ok || ens_particles_update(e, ctxs) != 0) {
     ens_particles_free(e);
@@ -1255,7 +1237,7 @@ void ens_particles_free(EnsParticles *e) {
 
 template <int PPT, int WAVES>
 void ens_launch_t(EnsParticles *e, int t, hipStream_t st) {
-  const int blocks = std::min(e->n_items[t], e->blocks[t]);
+  const int blocks = e->n_items[t];
   if (blocks <= 0) return;
   hipLaunchKernelGGL((k_ens_fused32<PPT, WAVES>), dim3(blocks), dim3(WAVES * 64), 0, st, e->args, e->sinks,
                      e->items[t], e->n_items[t]);

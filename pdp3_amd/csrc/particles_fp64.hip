@@ -686,11 +686,13 @@ __global__ void __launch_bounds__(256)
 k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s, long long n_cap, int pusher) {
   __shared__ double box[TileSink::kSize];
   const long long base = (long long)blockIdx.x * kBlockParticles;
-  // anchor: the block's first logical particle (smallest cell of a sorted block)
-  int ai = 0, ak = 0;
-  if (base < n_cap && s.alive[base] == 1) {
-    ai = max(0, cell_r(g, s.r[base]));
-    ak = max(0, cell_z(g, s.z[base]));
+  // anchor samples: slot base + L is the logical particle 16 L of the block
+  int ai, ak;
+  {
+    const int l = threadIdx.x & 63;
+    const bool live = l < 8 && base + l < n_cap && s.alive[base + l] == 1;
+    tile_anchor(live, live ? max(0, cell_r(g, s.r[base + l])) : 0, live ? max(0, cell_z(g, s.z[base + l])) : 0, ai,
+                ak);
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);

@@ -212,8 +212,42 @@ struct TileSinkT {
 using TileSink = TileSinkT<false>;
 using BunchSink = TileSinkT<true, 16, 32>;
 
+// The anchor cell of a tile of cell-sorted particles: the row that most of
+// its first particles are in, and the smallest column among those of them in
+// that row.  Lane L < 8 of the wavefront holds the cell (i, k) of the tile's
+// logical particle 16 L (live: a living particle); every lane gets the
+// result, (0, 0) when no sample lives.  Between sorts a few particles have
+// moved to a neighbouring cell: a box anchored at one that moved to the row
+// above or below would miss the whole tile (every particle of it would take
+// the boxes' HBM fallbacks), so the row is the lower median of the samples
+// (right with up to three strays among the eight).  A sample one column off
+// is harmless: the boxes keep two columns below the anchor.
+__device__ __forceinline__ void tile_anchor(bool live, int i, int k, int &ai, int &ak) {
+  const int lane = (int)__lane_id();
+  live = live && lane < 8;
+  const int row = live ? i : 0x7fffffff;
+  int m = 0, rank = 0;  // living samples; samples ordered before this one
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int rl = __shfl(row, l);
+    m += rl != 0x7fffffff;
+    rank += rl < row || (rl == row && l < lane);
+  }
+  const uint64_t med = __ballot(live && rank == (m - 1) / 2);
+  if (med == 0) {
+    ai = ak = 0;
+    return;
+  }
+  // (uniform values: scalar registers for the boxes' origins)
+  ai = __builtin_amdgcn_readfirstlane(__shfl(row, __ffsll((long long)med) - 1));
+  int col = live && i == ai ? k : 0x7fffffff;
+#pragma unroll
+  for (int l = 0; l < 8; l++) col = min(col, __shfl(col, l));
+  ak = __builtin_amdgcn_readfirstlane(col);
+}
+
 // block prologue: anchor the box one row and two columns below the given
-// cell (the block's smallest cell) and clear it
+// cell (the tile's anchor, or a bunch block's smallest cell) and clear it
 template <class T>
 __device__ __forceinline__ void box_init(double *box, int anchor_i, int anchor_k, T &s, const GlobalSink &g) {
   for (int t = threadIdx.x; t < T::kSize; t += blockDim.x) box[t] = 0.0;
