@@ -699,12 +699,19 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
                                              double *box, float *fbox) {
   static_assert(kFK == 64, "one wavefront per field box row");
   static_assert(kPPT % PPT == 0 && (WAVES * PPT) % kPPT == 0, "tile shape");
-  // anchor samples: slot base + L is the logical particle 16 L of the tile
+  // anchor samples: slot base + j is the logical particle 16 j of the tile
   int ai, ak;
   {
-    const int l = threadIdx.x & 63;
-    const uint32_t c = l < 8 && base + l < n_cap ? p.cell[base + l] : kDead;
-    tile_anchor(!(c & kDead), cell_i(c), cell_k(c), ai, ak);
+    bool live[3];
+    int ri[3], kk[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const uint32_t c = base + j < n_cap ? p.cell[base + j] : kDead;
+      live[j] = !(c & kDead);
+      ri[j] = cell_i(c);
+      kk[j] = cell_k(c);
+    }
+    tile_anchor(live, ri, kk, ai, ak);
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);

@@ -686,13 +686,18 @@ __global__ void __launch_bounds__(256)
 k_fused_tiled(GridDesc g, Fields f, GlobalSink gs, SpeciesArgs s, long long n_cap, int pusher) {
   __shared__ double box[TileSink::kSize];
   const long long base = (long long)blockIdx.x * kBlockParticles;
-  // anchor samples: slot base + L is the logical particle 16 L of the block
+  // anchor samples: slot base + j is the logical particle 16 j of the block
   int ai, ak;
   {
-    const int l = threadIdx.x & 63;
-    const bool live = l < 8 && base + l < n_cap && s.alive[base + l] == 1;
-    tile_anchor(live, live ? max(0, cell_r(g, s.r[base + l])) : 0, live ? max(0, cell_z(g, s.z[base + l])) : 0, ai,
-                ak);
+    bool live[3];
+    int ri[3], kk[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      live[j] = base + j < n_cap && s.alive[base + j] == 1;
+      ri[j] = live[j] ? max(0, cell_r(g, s.r[base + j])) : 0;
+      kk[j] = live[j] ? max(0, cell_z(g, s.z[base + j])) : 0;
+    }
+    tile_anchor(live, ri, kk, ai, ak);
   }
   TileSink sk;
   box_init(box, ai, ak, sk, gs);

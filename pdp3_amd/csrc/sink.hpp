@@ -212,38 +212,34 @@ struct TileSinkT {
 using TileSink = TileSinkT<false>;
 using BunchSink = TileSinkT<true, 16, 32>;
 
-// The anchor cell of a tile of cell-sorted particles: the row that most of
-// its first particles are in, and the smallest column among those of them in
-// that row.  Lane L < 8 of the wavefront holds the cell (i, k) of the tile's
-// logical particle 16 L (live: a living particle); every lane gets the
-// result, (0, 0) when no sample lives.  Between sorts a few particles have
-// moved to a neighbouring cell: a box anchored at one that moved to the row
-// above or below would miss the whole tile (every particle of it would take
-// the boxes' HBM fallbacks), so the row is the lower median of the samples
-// (right with up to three strays among the eight).  A sample one column off
-// is harmless: the boxes keep two columns below the anchor.
-__device__ __forceinline__ void tile_anchor(bool live, int i, int k, int &ai, int &ak) {
-  const int lane = (int)__lane_id();
-  live = live && lane < 8;
-  const int row = live ? i : 0x7fffffff;
-  int m = 0, rank = 0;  // living samples; samples ordered before this one
+// The anchor cell of a tile of cell-sorted particles, from the cells (i, k)
+// of its logical particles 0, 16 and 32 (live: a living particle; uniform
+// values, so that the choice runs on the scalar unit): the median row of the
+// living samples (the smaller when two live) and the smallest column among
+// the samples in that row; (0, 0) when none lives.  Between sorts a few
+// particles have moved to a neighbouring cell; a box anchored at one of them
+// that moved to another row would serve the tile's own row only at its edge,
+// so one such stray among the three samples is outvoted.  A sample one
+// column off is harmless: the boxes keep two columns below the anchor.
+__device__ __forceinline__ void tile_anchor(const bool (&live)[3], const int (&i)[3], const int (&k)[3], int &ai,
+                                            int &ak) {
+  constexpr int kNone = 0x7fffffff;
+  int r[3];
 #pragma unroll
-  for (int l = 0; l < 8; l++) {
-    const int rl = __shfl(row, l);
-    m += rl != 0x7fffffff;
-    rank += rl < row || (rl == row && l < lane);
-  }
-  const uint64_t med = __ballot(live && rank == (m - 1) / 2);
-  if (med == 0) {
+  for (int j = 0; j < 3; j++) r[j] = live[j] ? i[j] : kNone;
+  const int n = (int)live[0] + (int)live[1] + (int)live[2];
+  if (n == 0) {
     ai = ak = 0;
     return;
   }
-  // (uniform values: scalar registers for the boxes' origins)
-  ai = __builtin_amdgcn_readfirstlane(__shfl(row, __ffsll((long long)med) - 1));
-  int col = live && i == ai ? k : 0x7fffffff;
+  const int lo = min(min(r[0], r[1]), r[2]);
+  ai = lo;
+  if (n == 3) ai = r[0] + r[1] + r[2] - lo - max(max(r[0], r[1]), r[2]);  // the median (rows are small)
+  int col = kNone;
 #pragma unroll
-  for (int l = 0; l < 8; l++) col = min(col, __shfl(col, l));
-  ak = __builtin_amdgcn_readfirstlane(col);
+  for (int j = 0; j < 3; j++)
+    if (live[j] && i[j] == ai) col = min(col, k[j]);
+  ak = col;
 }
 
 // block prologue: anchor the box one row and two columns below the given
