@@ -536,6 +536,9 @@ def main():
         "speedup_fp32_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "error_fp32_vs_fp64": err,
         "roofline": roofline,
+        # device time per timed step by kernel group (HIP events on the
+        # library's stream); the rest of ms_per_step is resets and launch gaps
+        "device_ms_per_step": {k: r32[k + "_ms"] / a.steps for k in ("particle", "bunch", "h", "e", "sort")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

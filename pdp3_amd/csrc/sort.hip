@@ -332,7 +332,8 @@ __global__ void __launch_bounds__(256) k_run_place(K keys, long long nch, const 
 // its chunk of src coalesced and transposes it to slot order in LDS (rows
 // padded to 17 words); each thread then moves kGatherPer particles (256
 // slots apart) so that all their loads are in flight together (non-temporal
-// loads were slower: the slots of neighbouring particles share cache lines).
+// loads were slower: the slots of neighbouring particles share cache lines;
+// the stores, each written once, are non-temporal: 5% off the sort).
 struct Arrays7 {
   void *a[7];
 };
@@ -344,7 +345,7 @@ __device__ __forceinline__ void gather_array(const void *in, void *out, const ui
 #pragma unroll
   for (int u = 0; u < kGatherPer; u++) x[u] = static_cast<const T *>(in)[q[u]];
 #pragma unroll
-  for (int u = 0; u < kGatherPer; u++) static_cast<T *>(out)[p0 + u * 256] = x[u];
+  for (int u = 0; u < kGatherPer; u++) __builtin_nontemporal_store(x[u], static_cast<T *>(out) + p0 + u * 256);
 }
 // the source slots of this thread's kGatherPer output slots c0 + u*256 + tid
 __device__ __forceinline__ void gather_sources(const uint32_t *__restrict__ src, long long c0,
