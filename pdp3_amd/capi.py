@@ -423,7 +423,7 @@ def adjoint_run_batch(sims, nsteps, checkpoint_every, window_steps, z_lo, z_hi):
 
 class Ensemble:
     """Independent FP32-form simulations advanced together (one graph replay
-    per step with one persistent particle launch over every instance)."""
+    per step with one particle launch over every instance)."""
 
     def __init__(self, sims):
         self.L = lib()
@@ -432,7 +432,7 @@ class Ensemble:
         h = _P()
         st = self.L.pdp3_ensemble_create(arr, len(self.sims), C.byref(h))
         if st != 0:
-            raise Pdp3Error(st, "pdp3_ensemble_create failed (FP32 contexts, one dt, loaded bunches only)")
+            raise Pdp3Error(st, "pdp3_ensemble_create failed (FP32 contexts with one common dt)")
         self.h = h
 
     def _chk(self, st):

@@ -292,7 +292,10 @@ int sort_prepare(Ctx &c, Species &sp);
 struct MultiSeg {
   const uint32_t *cell;
   long long slot_base;
-  uint32_t key_base, ncell, nzm1;
+  // keys [key_base, key_base + ncell) for the living particles (ncell: the
+  // segment's number of live keys), then one for dead ones and one for
+  // padding; band: cells ordered in bands of two rows, column by column
+  uint32_t key_base, ncell, nzm1, band;
   void *in[7], *out[7];
 };
 struct MultiSort {

@@ -431,15 +431,17 @@ struct KeysF64 {
 __device__ __forceinline__ int seg_of(const MultiSeg *seg, const uint32_t *seg_of_chunk, long long j) {
   return (int)seg_of_chunk[j / kChunk];
 }
-// the keys of one segment (global slot j)
+// the keys of one segment (global slot j): row-major cells, or bands of two
+// rows ordered column by column (cell (i, k) -> (i / 2) 2 (nz-1) + 2 k + i % 2)
 struct KeysSeg {
   const uint32_t *cell;  // shifted so that cell[j] is global slot j
-  uint32_t key_base, ncell, nzm1;
+  uint32_t key_base, ncell, nzm1, band;
   __device__ __forceinline__ uint32_t operator()(long long j) const {
     const uint32_t c = cell[j];
     if (c == kPadCell) return key_base + ncell + 1;
     if (c & 0x80000000u) return key_base + ncell;
-    return key_base + ((c >> 16) & 0x7FFF) * nzm1 + (c & 0xFFFF);
+    const uint32_t i = (c >> 16) & 0x7FFF, k = c & 0xFFFF;
+    return key_base + (band ? (i >> 1) * (2 * nzm1) + 2 * k + (i & 1) : i * nzm1 + k);
   }
 };
 struct KeysMulti {
@@ -449,7 +451,7 @@ struct KeysMulti {
   __device__ __forceinline__ KeysSeg at_chunk(long long j0) const {
     const int s = __builtin_amdgcn_readfirstlane(seg_of(seg, seg_of_chunk, j0));
     const MultiSeg &d = seg[s];
-    return KeysSeg{d.cell - d.slot_base, d.key_base, d.ncell, d.nzm1};
+    return KeysSeg{d.cell - d.slot_base, d.key_base, d.ncell, d.nzm1, d.band};
   }
   __device__ __forceinline__ uint32_t operator()(long long j) const { return at_chunk(j - j % kChunk)(j); }
 };
@@ -709,8 +711,14 @@ int sort_multi(MultiSort &ms, const std::vector<Ctx *> &cs, hipStream_t stream) 
     d.cell = (const uint32_t *)sp.f32.cell;
     d.slot_base = n;
     d.key_base = (uint32_t)keys;
-    d.ncell = (uint32_t)(c.g.nr - 1) * (uint32_t)(c.g.nz - 1);
     d.nzm1 = (uint32_t)(c.g.nz - 1);
+    // A particle tile (a few chunks of consecutive particles) spans
+    // tile / ppc cells; in row order that overruns the tile's 64-column field
+    // box below about 32 particles per cell, in bands of two rows only below
+    // about 16.  Instances with fewer particles per cell sort in bands.
+    const long long rows = c.g.nr - 1;
+    d.band = sp.n < 32LL * rows * (long long)d.nzm1 ? 1u : 0u;
+    d.ncell = d.band ? (uint32_t)(2 * ((rows + 1) / 2)) * d.nzm1 : (uint32_t)rows * d.nzm1;
     SpeciesF32 &a = sp.f32, &b = sp.f32_alt;
     void *ap[7] = {a.cell, a.fr, a.fz, a.vr, a.vphi, a.vz, a.w};
     void *bp[7] = {b.cell, b.fr, b.fz, b.vr, b.vphi, b.vz, b.w};

@@ -84,6 +84,47 @@ def test_ensemble_tiles_and_autotune():
     ens.close()
 
 
+def test_ensemble_sort_orders_by_cell_or_band():
+    """The ensemble's batched sort orders each instance's plasma stably by its
+    key: row-major cells for instances with at least 32 particles per cell,
+    bands of two rows column by column below that.  Cold particles with no
+    fields stay where they are, so two steps (one sort each) leave the
+    shuffled initial state in exactly the sorted order."""
+    from pdp3_amd import capi
+
+    rng = np.random.default_rng(41)
+    dr = 5e-4
+    shapes = [(16, 64, 6), (9, 40, 40), (33, 20, 3)]  # cells r, cells z, particles per cell
+    sims, before, keys = [], [], []
+    for cr, cz, ppc in shapes:
+        sim = capi.Simulation(cr + 1, cz + 1, cr * dr, cz * dr, 0.5 * dr / 3e8, precision=capi.PREC_FP32)
+        n = cr * cz * ppc
+        # inside the reflecting walls at dr/2 from the axis and the boundaries
+        r = rng.uniform(0.51, cr - 0.51, n) * dr
+        z = rng.uniform(0.51, cz - 0.51, n) * dr
+        r[: n // 8] = (rng.integers(1, cr - 1, n // 8) + 0.25) * dr  # many particles sharing cells
+        zero = np.zeros(n)
+        sim.add_plasma(-1, 1.0, r, z, zero, zero, zero, np.full(n, 1e3))
+        st = sim.download(0)
+        # the cells the library packed the particles into (CELL_NUMBER of the input)
+        i = np.clip(np.ceil(r / sim.dr).astype(np.int64) - 1, 0, cr - 1)
+        k = np.clip(np.ceil(z / sim.dz).astype(np.int64) - 1, 0, cz - 1)
+        band = n < 32 * cr * cz
+        keys.append((i // 2) * 2 * cz + 2 * k + (i & 1) if band else i * cz + k)
+        before.append(st)
+        sims.append(sim)
+    ens = capi.Ensemble(sims)
+    ens.set_sort_interval(1)
+    ens.step(2)
+    ens.close()
+    for sim, st, key in zip(sims, before, keys):
+        after = sim.download(0)
+        order = np.argsort(key, kind="stable")
+        for name in ("r", "z", "vr", "vphi", "vz", "w"):
+            assert np.array_equal(after[name], st[name][order]), name
+        sim.close()
+
+
 def test_ensemble_rejects_fp64():
     from pdp3_amd import capi
     from pdp3_amd import workloads as wl
