@@ -256,9 +256,9 @@ int init_beam(pdp3_ctx *ctx, bool inject = true) {
   return PDP3_OK;
 }
 
-int zero_grid(pdp3_ctx *ctx, int comp) {
+int zero_grid(pdp3_ctx *ctx, int comp, int count = 1) {
   Ctx &c = ctx->c;
-  launch_fill(c.a[comp], c.cells(), 0.0, c.stream);
+  launch_fill(c.a[comp], c.cells() * (size_t)count, 0.0, c.stream);  // the components are contiguous
   CHECK_LAUNCH(ctx);
   return PDP3_OK;
 }
@@ -982,8 +982,10 @@ int step_one(pdp3_ctx *ctx, const std::function<int()> *after_h) {
   if ((st = init_beam(ctx))) return st;
   if ((st = pdp3_hfield_calc(ctx))) return st;
   if (after_h && (st = (*after_h)())) return st;
-  if ((st = pdp3_current_reset(ctx))) return st;
-  if ((st = zero_grid(ctx, kRhoBunch))) return st;
+  // Current::reset_j and the bunch rho's reset in one launch (J_r, J_phi,
+  // J_z and rho_bunch are consecutive components of the grid)
+  static_assert(kJPhi == kJR + 1 && kJZ == kJR + 2 && kRhoBunch == kJR + 3, "grid component order");
+  if ((st = zero_grid(ctx, kJR, 4))) return st;
   for (int j : c.order)
     if ((st = particles_fused(ctx, c.species[j]))) return st;
   if ((st = pdp3_efield_calc(ctx))) return st;
