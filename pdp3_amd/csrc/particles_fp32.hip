@@ -699,7 +699,8 @@ __device__ __forceinline__ void fused32_tile(const F32Args &p, const GlobalSink 
                                              double *box, float *fbox) {
   static_assert(kFK == 64, "one wavefront per field box row");
   static_assert(kPPT % PPT == 0 && (WAVES * PPT) % kPPT == 0, "tile shape");
-  // anchor samples: slot base + j is the logical particle 16 j of the tile
+  // anchor samples: slot base + j is the logical particle 16 j of a plasma
+  // tile (the chunk-transposed layout), particle j of an ensemble's bunch block
   int ai, ak;
   {
     bool live[3];
