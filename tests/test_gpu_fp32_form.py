@@ -12,7 +12,8 @@ cannot be bit-compatible with the reference; its contract is accuracy:
     rather than a field's average) at most 1e-2.
 The reduced copies of cfg2, cfg3 and cfg4 are compared with the oracle in
 test_gpu_configs.py; bench.py reports cfg3's E_z error after its 1000 steps
-on the full grid against the GPU's FP64 plain form.
+on the full grid against the GPU's FP64 form with the strict-radial
+precedence fix, and against the plain FP64 form beside it.
 """
 import numpy as np
 import pytest
