@@ -251,7 +251,7 @@ def traffic_from_profile(name):
 def bench_ensemble(a):
     """cfg4: 128 independent instances (per-instance grid, density, ppc, bunch
     ratio drawn from the seed), FP32 form, one graph replay per step with one
-    persistent particle launch; the tile shape picked by the autotuner."""
+    particle launch; the tile shape picked by the autotuner."""
     from pdp3_amd import capi
     from pdp3_amd import workloads as wl
 

@@ -12,7 +12,7 @@
   relative RMS error against the oracle after 100 steps <= 1e-4, the
   precision target of the configuration;
 * cfg4: each member of a three-instance ensemble of uneven grids (one graph
-  replay per step, one persistent particle launch) against the oracle run of
+  replay per step, one particle launch) against the oracle run of
   that member alone, FP32 E_z relative RMS <= 1e-4 after 60 steps.
 
 The FP32 runs start from the FP64 initial state rounded to the FP32 form

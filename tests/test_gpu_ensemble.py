@@ -1,5 +1,5 @@
 """Ensembles: independent instances of different shapes advanced together
-(graph replay, one persistent particle launch) give the same result as each
+(graph replay, one particle launch) give the same result as each
 instance stepped alone, with and without periodic cell sorts, for every tile
 shape the autotuner may pick, with bunches loaded at the start or injected
 over time (Bunch::bunch_inject from the device step counter)."""
