@@ -1167,14 +1167,21 @@ struct EnsParticles {
   int n_args = 0;
   F32Args *args = nullptr;
   GlobalSink *sinks = nullptr;
-  // work list per tile shape (plasma tiles and bunch blocks)
-  EnsItem *items[kNumTiles] = {};
-  int n_items[kNumTiles] = {};
+  // work list per tile shape (plasma tiles and bunch blocks), and at index
+  // kNumTiles the one-chunk tiles of the sparse plasma species (below
+  // kSparsePpc particles per cell), which the lists of the larger shapes leave
+  // out: a tile spans tile / ppc cells, and one chunk is the smallest tile
+  EnsItem *items[kNumTiles + 1] = {};
+  int n_items[kNumTiles + 1] = {};
 };
 
 namespace {
+constexpr int kSparsePpc = 16;
+constexpr int kChunkTile = kNumTiles - 1;  // 4 x 4: one chunk per tile
+static_assert(kTileShapes[kChunkTile][0] * kTileShapes[kChunkTile][1] == kPPT, "one-chunk tile shape");
 // the argument table and the work lists of the ensemble's species
-void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::vector<EnsItem> items[kNumTiles]) {
+void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args,
+               std::vector<EnsItem> items[kNumTiles + 1]) {
   int a = 0;
   for (Ctx *cp : ctxs) {
     Ctx &c = *cp;
@@ -1184,8 +1191,14 @@ void ens_lists(const std::vector<Ctx *> &ctxs, std::vector<F32Args> *args, std::
       Species &sp = c.species[j];
       if (sp.n_cap <= 0 || !sp.f32.cell) continue;
       args->push_back(args32(c, sp));
-      for (int t = 0; t < kNumTiles; t++) {
-        const long long tile = (long long)kTileShapes[t][1] * kTileShapes[t][0] / kPPT * kChunk;
+      const bool sparse = sp.chunked && sp.n < (long long)kSparsePpc * (c.g.nr - 1) * (c.g.nz - 1);
+      for (int t = 0; t <= kNumTiles; t++) {
+        // the one-chunk shape's own list holds every species; the others
+        // leave the sparse ones to the extra list
+        if (t < kNumTiles && t != kChunkTile && sparse) continue;
+        if (t == kNumTiles && !sparse) continue;
+        const int ts = t == kNumTiles ? kChunkTile : t;
+        const long long tile = (long long)kTileShapes[ts][1] * kTileShapes[ts][0] / kPPT * kChunk;
         for (long long b = 0; b < sp.n_cap; b += tile) items[t].push_back(EnsItem{a, sp.chunked ? 0 : 1, b, sp.n_cap});
       }
       a++;
@@ -1216,12 +1229,12 @@ int ens_particles_update(EnsParticles *e, const std::vector<Ctx *> &ctxs) {
 EnsParticles *ens_particles_build(const std::vector<Ctx *> &ctxs) {
   auto *e = new EnsParticles;
   std::vector<F32Args> args;
-  std::vector<EnsItem> items[kNumTiles];
+  std::vector<EnsItem> items[kNumTiles + 1];
   ens_lists(ctxs, &args, items);
   e->n_args = (int)args.size();
   bool ok = hipMalloc((void **)&e->args, std::max(1, e->n_args) * sizeof(F32Args)) == hipSuccess &&
             hipMalloc((void **)&e->sinks, std::max(1, e->n_args) * sizeof(GlobalSink)) == hipSuccess;
-  for (int t = 0; ok && t < kNumTiles; t++) {
+  for (int t = 0; ok && t <= kNumTiles; t++) {
     e->n_items[t] = (int)items[t].size();
     ok = hipMalloc((void **)&e->items[t], std::max<size_t>(1, items[t].size()) * sizeof(EnsItem)) == hipSuccess &&
          (items[t].empty() || hipMemcpy(e->items[t], items[t].data(), items[t].size() * sizeof(EnsItem),
@@ -1271,6 +1284,9 @@ void ens_launch_particles(EnsParticles *e, int tile, hipStream_t st) {
     case 4: ens_launch_t<8, 2>(e, 4, st); break;
     default: ens_launch_t<4, 4>(e, 5, st); break;
   }
+  // the sparse species on one-chunk tiles (their list is empty when the
+  // shape is the one-chunk shape itself, whose list holds them)
+  if (tile != kChunkTile) ens_launch_t<4, 4>(e, kNumTiles, st);
 }
 
 }  // namespace pdp3
